@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Headline benchmark: MC swap attempts/s summed over all replicas, parallel-tempering CE swap Monte Carlo.
+
+Workload (default, the 1,048,576-site configuration): fcc 64 x 64 x 64 conventional cells = 1,048,576 sites, Ni/Pt
+with Pt fraction x = 0.25 at random from a seed, cluster space pairs <= 8 A / triplets <= 6 A / quadruplets <= 5 A,
+synthetic ECIs N(0, (20 meV / r^2)^2), 256 replicas on a geometric ladder 300-1400 K, int8 occupations, FP32 swap
+energy change with FP64 running totals / exchange / accumulators ("mixed"), exchange every 5000 steps.
+
+One bench step = one chunk of --chunk MC steps of every replica (one nce_pt_run_chunk launch) followed by the
+temperature-swap exchange (one nce_pt_exchange launch).  Inputs are resident in HBM before the timed region.
+
+The line also carries
+  roofline      the chunk kernel's algorithmic HBM bytes (2 covering windows of 141 int8 sites read + 2 sites written
+                = 284 B per attempt) over its average launch duration (HIP events on the launch stream), against
+                MI355X's 8 TB/s;
+  cpu_baseline  the FP64 C oracle (same proposal / swap energy change / Metropolis) timed on this host's cores on a
+                bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_ALG_PER_ATTEMPT = 284           # bytes: two 141-site int8 windows + two written sites (8/6/5 fcc space)
+KB = 8.617333e-5
+PUBLISHED_H100 = 4_605_830.0      # reference: CE attempts/s at N = 1,048,576, R = 256 on H100 (pure step loop)
+
+CONFIGS = {
+    "ce_1m": dict(n=64, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
+    "ce_2k": dict(n=8, x=0.5, R=32, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="fp64"),
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed chunks")
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--chunk", type=int, default=5000, help="MC steps per replica per chunk")
+    ap.add_argument("--config", default="ce_1m", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"])
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the cpu_baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(info, occ0, betas, target_s, seed):
+    """FP64 oracle on this host's cores: bounded sample (all replicas, n steps sized to ~target_s)."""
+    from oracle.ce_oracle import OracleCE
+    o = OracleCE(info)
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    R = occ0.shape[0]
+    occ = np.ascontiguousarray(occ0.copy())
+    E = np.array([o.energy(occ[0])] * R)
+    t = time.perf_counter()
+    o.run_chunk(occ, E, betas, 4, seed=seed, step0=10**9, n_threads=cores)
+    dt = time.perf_counter() - t
+    per_step = max(dt / 4, 1e-6)
+    n = int(max(4, min(100000, target_s / per_step)))
+    t = time.perf_counter()
+    o.run_chunk(occ, E, betas, n, seed=seed, step0=2 * 10**9, n_threads=cores)
+    dt = time.perf_counter() - t
+    return dict(value=R * n / dt, unit="MC swap attempts/s", cores=cores, kind="port",
+                sample=f"{R} replicas x {n} steps of the same cell and ladder, FP64 oracle, OpenMP over replicas "
+                       f"({dt:.1f} s)")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is measured per GPU", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from neuralce2_amd.synthetic import synthetic_ce, fcc_cell
+    from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
+    from neuralce2_amd._lib import PREC_FP64, PREC_MIXED
+    cfg = dict(CONFIGS[args.config])
+    if args.precision:
+        cfg["precision"] = args.precision
+    prec = PREC_MIXED if cfg["precision"] == "mixed" else PREC_FP64
+    ce = synthetic_ce(args.seed)
+    at = fcc_cell(cfg["n"], cfg["x"], args.seed)
+    g = GPUClusterExpansion(ce, at, energy_scale=float(len(at)))
+    R = cfg["R"]
+    temps = np.geomspace(cfg["Tmin"], cfg["Tmax"], R).astype(np.float32)
+    betas = (1.0 / (KB * temps.astype(np.float64))).astype(np.float32).astype(np.float64)
+    sig = g.encode(at).astype(np.int8)
+    chunk = args.chunk
+    st = PTJobState(g, sig, betas, seed=args.seed, sample_interval=chunk, samples_per_chunk=1)
+    stream = torch.cuda.current_stream()
+    ex_every_chunks = max(1, cfg["exchange"] // chunk)
+
+    def one(c, ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        g.ctx.pt_run_chunk([st.job(accumulate=True)], chunk, prec, args.tile)
+        if ev is not None:
+            ev[1].record(stream)
+        if (c + 1) % ex_every_chunks == 0:
+            g.ctx.pt_exchange([st.exchange_job(c + 1, reorder_last=True)])
+
+    for c in range(args.warmup):
+        one(c)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in range(args.steps):
+        one(args.warmup + c, evs[c])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    k_ms = np.array([a.elapsed_time(b) for a, b in evs])
+    attempts = R * chunk * args.steps
+    value = attempts / dt
+    bytes_launch = R * chunk * B_ALG_PER_ATTEMPT
+    achieved = bytes_launch / (k_ms.mean() * 1e-3) / 1e9
+    line = {
+        "metric": "MC swap attempts/s (all replicas)",
+        "value": value,
+        "unit": "attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_H100 if args.config == "ce_1m" else None,
+        "dtype": "f32 dE / f64 totals" if prec == PREC_MIXED else "f64",
+        "data": "synthetic (seeded random occupations and ECIs)",
+        "config": {"workload": f"CE PT swap MC, fcc {cfg['n']}^3 cubic = {len(at)} sites, x={cfg['x']}, "
+                               f"{R} replicas, T {cfg['Tmin']:.0f}-{cfg['Tmax']:.0f} K, cutoffs 8/6/5 A, "
+                               f"exchange every {cfg['exchange']} steps",
+                   "sites": len(at), "replicas": R, "chunk_steps": chunk, "precision": cfg["precision"],
+                   "tile": args.tile},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "ce_chunk_kernel", "kernel_ms_avg": float(k_ms.mean()),
+                     "bytes_per_attempt_alg": B_ALG_PER_ATTEMPT},
+        "cpu_baseline": None,
+        "acceptance": float(st.n_accept.sum().item()) / (R * chunk * (args.warmup + args.steps)),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(g.info, np.broadcast_to(sig, (R, len(sig))), betas,
+                                            args.cpu_seconds, args.seed)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/* neuralce_hip.h — C ABI of libneuralce_hip.so, the gfx950 implementation of neuralce's parallel-tempering
+ * canonical-swap Monte Carlo.
+ *
+ * The reference's boundary is a Python/JAX API, not an FFI; each entry point below replaces the JAX function named
+ * in its comment (paths relative to the reference repository).  Conventions:
+ *   - every call returns 0 on success or a negative error code; nce_last_error() describes the last failure of the
+ *     calling thread;
+ *   - buffers passed to compute calls are DEVICE pointers owned by the caller (torch tensors' data_ptr());
+ *     the library owns only the context and the uploaded model tables;
+ *   - compute calls are stream-ordered on `stream` (a hipStream_t, NULL = default stream) and do not synchronise;
+ *   - one context per host thread; a context has no internal locking.
+ * Sites are in canonical order i = cell * n_basis + basis, cell = (k0 d1 + k1) d2 + k2 (k = Smith-normal-form cell
+ * coordinate), exactly as neuralce/ce/ce_gpu_implicit.py:1-21.
+ */
+#ifndef NEURALCE_HIP_H
+#define NEURALCE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NCE_OK 0
+#define NCE_ERR_ARG (-1)
+#define NCE_ERR_HIP (-2)
+#define NCE_ERR_UNSUPPORTED (-3)
+#define NCE_ERR_NOMEM (-4)
+
+#define NCE_PREC_FP64 0      /* plain form: FP64 swap energy change */
+#define NCE_PREC_MIXED 1     /* FP32 swap energy change, FP64 running totals / exchange / accumulators */
+
+typedef struct nce_ctx nce_ctx;
+
+/* Cluster-expansion model on one cell shape: cell-independent template (reverse entries grouped into per-basis
+ * windows, forward rows) plus the cell's reduced offsets.  Host arrays; copied to the device by nce_ce_model_create.
+ * Replaces the `tp` pytree built by build_ce_gpu_implicit (neuralce/ce/ce_gpu_implicit.py:50-126). */
+typedef struct {
+    int32_t n_basis;          /* nB */
+    int32_t max_order;        /* mo <= 4 */
+    int32_t n_entries;        /* J: reverse entries per basis (padded) */
+    int32_t max_window;       /* Wmax: window slots per basis (padded) */
+    int32_t n_rows;           /* Rt: forward rows */
+    int32_t n_sublattices;
+    int32_t n_active;         /* active sublattices (proposal groups) */
+    int32_t max_active_basis;
+    int32_t d[3];             /* Smith normal form diagonal of the cell */
+    int64_t n_cells;
+    double factor;            /* energy_scale / n_cells */
+    double zerolet;           /* zerolet * energy_scale */
+    const int32_t *entries_per_basis;  /* [nB] */
+    const int32_t *window_per_basis;   /* [nB] */
+    const uint16_t *entry_slot;        /* [nB][J][4] window slot of each member; 0xFFFF = padding (value 1) */
+    const double *entry_coef;          /* [nB][J] eci_orbit / n_instances_orbit; 0 for padding entries */
+    const int32_t *window;             /* [nB][Wmax][4] (basis, k0, k1, k2): U*offset mod d */
+    const int32_t *rows;               /* [Rt][mo][4] forward rows (basis, U*offset mod d), padding = anchor */
+    const int32_t *row_bf;             /* [Rt][mo] point-function index (0 = constant 1) */
+    const double *row_coef;            /* [Rt] */
+    const double *pf;                  /* [n_sublattices][2][2] point functions pf[sl][bf][species] */
+    const int32_t *basis_to_sl;        /* [nB] */
+    const int32_t *active_basis;       /* [n_active][max_active_basis] */
+    const int32_t *active_nbasis;      /* [n_active] */
+    const double *active_cum;          /* [n_active] cumulative selection probability */
+} nce_ce_model_desc;
+
+/* One parallel-tempering job for nce_pt_run_chunk.  All pointers are DEVICE pointers. */
+typedef struct {
+    int32_t model;              /* id returned by nce_ce_model_create */
+    int32_t n_replicas;         /* R */
+    int64_t n_sites;            /* N (= n_cells * n_basis of the model) */
+    int8_t *occ;                /* [R][N] species index per site, canonical order */
+    double *energy;             /* [R] running total energies */
+    const double *beta;         /* [R] 1/(kB T) of each replica's current temperature slot */
+    const int32_t *slot_of_rep; /* [R] */
+    int64_t *step;              /* [R] global step counter (Philox counter), advanced by n_steps */
+    uint64_t seed;              /* Philox key of the job */
+    int64_t *n_accept;          /* [R] accepted swaps, accumulated */
+    double *e_min;              /* [R] running minimum energy */
+    double *traj;               /* [n_steps / sample_interval][R] energy by slot after every interval, or NULL */
+    int32_t sample_interval;
+    int32_t accumulate;         /* 1: add (E - e_ref) and (E - e_ref)^2 of every step to s1, s2 */
+    const double *e_ref;        /* [R] */
+    double *s1, *s2;            /* [R] */
+    const int64_t *inj_ia;      /* [R][n_steps] injected proposals (site a), or NULL for the Philox stream */
+    const int64_t *inj_ib;      /* [R][n_steps] */
+    const double *inj_u;        /* [R][n_steps] Metropolis uniforms */
+} nce_pt_job;
+
+/* Temperature-swap exchange of one job (device pointers). */
+typedef struct {
+    int32_t n_replicas;
+    int32_t inj_parity;         /* >= 0: injected parity, < 0: Philox */
+    const double *energy;       /* [R] per replica */
+    int32_t *rep_of_slot;       /* [R] updated */
+    int32_t *slot_of_rep;       /* [R] updated */
+    const double *beta_slot;    /* [R] ladder */
+    const int32_t *ladder_id;   /* [R] or NULL */
+    double *beta_rep;           /* [R] updated: beta of each replica's new slot */
+    uint64_t seed;
+    uint64_t ex_index;          /* exchange counter (Philox counter) */
+    int64_t *n_swaps;           /* [1] accumulated accepted exchanges */
+    const double *inj_u;        /* [ceil((R-1)/2)] injected uniforms or NULL */
+    double *traj_last;          /* [R] if not NULL: rewritten as energy[rep_of_slot[t]] after the exchange */
+} nce_pt_exchange_job;
+
+const char *nce_last_error(void);
+
+/* Context on device `device_ordinal` (replaces JAX's device selection; neuralce/mc/run.py:242-245). */
+int nce_ctx_create(int32_t device_ordinal, nce_ctx **out);
+void nce_ctx_destroy(nce_ctx *ctx);
+
+/* Upload a CE model (template + cell); replaces GPUClusterExpansion's implicit branch
+ * (neuralce/ce/classical_ce_gpu.py:514-535) / build_ce_gpu_implicit (neuralce/ce/ce_gpu_implicit.py:50-126). */
+int nce_ce_model_create(nce_ctx *ctx, const nce_ce_model_desc *desc, int32_t *model_id);
+
+/* Full energies of R configurations: E[r] = zerolet + factor * sum_cells sum_rows coef * prod pf.
+ * Replaces energy_a (neuralce/ce/ce_gpu_implicit.py:147-163) and its lax.map over replicas
+ * (neuralce/ce/pt_mcmc_ce.py:845-846). */
+int nce_ce_energy(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_replicas, int64_t n_sites,
+                  double *energy, void *stream);
+
+/* Swap energy change of explicit proposals (one per replica): dE[r] for swapping ia[r] and ib[r] of occ[r].
+ * Replaces delta_energy_a (neuralce/ce/ce_gpu_implicit.py:181-191). */
+int nce_ce_delta(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_replicas, int64_t n_sites,
+                 const int64_t *ia, const int64_t *ib, double *dE, int32_t precision, void *stream);
+
+/* n_steps swap-Metropolis steps of every replica of every job, one launch (persistent within the chunk).
+ * Replaces run_chunk = lax.scan(micro) of vmapped delta mc_step (neuralce/ce/pt_mcmc_ce.py:554-566,765-818) with
+ * make_fast_swap_implicit proposals (neuralce/ce/ce_gpu_implicit.py:194-211).
+ * jobs: HOST array of n_jobs job descriptors (copied to the device stream-ordered).
+ * tile: 0 = autotuned default, else waves per replica (1, 2, 4 or 8). */
+int nce_pt_run_chunk(nce_ctx *ctx, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps, int32_t precision,
+                     int32_t tile, void *stream);
+
+/* Temperature-swap exchange of each job; replaces exchange_slots (neuralce/ce/pt_mcmc_ce.py:748-760) and the
+ * reordering of the last sample after it (:808-810).  jobs: HOST array. */
+int nce_pt_exchange(nce_ctx *ctx, const nce_pt_exchange_job *jobs, int32_t n_jobs, void *stream);
+
+/* Philox4x32-10 of the library (known-answer checks). */
+void nce_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""ctypes binding of libneuralce_hip.so (declarations in include/neuralce_hip.h).
+
+The shared library is the only compute path: importing this module on a machine where the library is missing or
+cannot be loaded raises immediately; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("NCE_HIP_LIB", os.path.join(_HERE, "libneuralce_hip.so"))
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+_p = C.c_void_p
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("n_basis", C.c_int32), ("max_order", C.c_int32), ("n_entries", C.c_int32),
+                ("max_window", C.c_int32), ("n_rows", C.c_int32), ("n_sublattices", C.c_int32),
+                ("n_active", C.c_int32), ("max_active_basis", C.c_int32), ("d", C.c_int32 * 3),
+                ("n_cells", C.c_int64), ("factor", C.c_double), ("zerolet", C.c_double),
+                ("entries_per_basis", _p), ("window_per_basis", _p), ("entry_slot", _p), ("entry_coef", _p),
+                ("window", _p), ("rows", _p), ("row_bf", _p), ("row_coef", _p), ("pf", _p), ("basis_to_sl", _p),
+                ("active_basis", _p), ("active_nbasis", _p), ("active_cum", _p)]
+
+
+class PTJob(C.Structure):
+    _fields_ = [("model", C.c_int32), ("n_replicas", C.c_int32), ("n_sites", C.c_int64), ("occ", _p),
+                ("energy", _p), ("beta", _p), ("slot_of_rep", _p), ("step", _p), ("seed", C.c_uint64),
+                ("n_accept", _p), ("e_min", _p), ("traj", _p), ("sample_interval", C.c_int32),
+                ("accumulate", C.c_int32), ("e_ref", _p), ("s1", _p), ("s2", _p), ("inj_ia", _p), ("inj_ib", _p),
+                ("inj_u", _p)]
+
+
+class ExchangeJob(C.Structure):
+    _fields_ = [("n_replicas", C.c_int32), ("inj_parity", C.c_int32), ("energy", _p), ("rep_of_slot", _p),
+                ("slot_of_rep", _p), ("beta_slot", _p), ("ladder_id", _p), ("beta_rep", _p), ("seed", C.c_uint64),
+                ("ex_index", C.c_uint64), ("n_swaps", _p), ("inj_u", _p), ("traj_last", _p)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                          f"or neuralce2_amd/csrc/build.sh")
+    lib = C.CDLL(LIB_PATH)
+    lib.nce_last_error.restype = C.c_char_p
+    lib.nce_ctx_create.argtypes = [C.c_int32, C.POINTER(_p)]
+    lib.nce_ctx_destroy.argtypes = [_p]
+    lib.nce_ctx_destroy.restype = None
+    lib.nce_ce_model_create.argtypes = [_p, C.POINTER(ModelDesc), C.POINTER(C.c_int32)]
+    lib.nce_ce_energy.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p]
+    lib.nce_ce_delta.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p, _p, C.c_int32, _p]
+    lib.nce_pt_run_chunk.argtypes = [_p, _p, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _p]
+    lib.nce_pt_exchange.argtypes = [_p, _p, C.c_int32, _p]
+    lib.nce_philox4x32_10.argtypes = [_p, _p, _p]
+    lib.nce_philox4x32_10.restype = None
+    return lib
+
+
+lib = _load()
+
+EXPORTED = ("nce_last_error", "nce_ctx_create", "nce_ctx_destroy", "nce_ce_model_create", "nce_ce_energy",
+            "nce_ce_delta", "nce_pt_run_chunk", "nce_pt_exchange", "nce_philox4x32_10")
+
+
+class NCEError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise NCEError(f"libneuralce_hip error {rc}: {lib.nce_last_error().decode()}")
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32); k = np.ascontiguousarray(key, np.uint32); o = np.zeros(4, np.uint32)
+    lib.nce_philox4x32_10(c.ctypes.data, k.ctypes.data, o.ctypes.data)
+    return o
+
+
+def ptr(t):
+    """device (or host) address of a torch tensor / numpy array, 0 for None."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+class Context:
+    """One device context (nce_ctx); owns uploaded models."""
+
+    _by_device = {}
+
+    def __init__(self, device=0):
+        h = _p()
+        check(lib.nce_ctx_create(int(device), C.byref(h)))
+        self.h = h
+        self.device = int(device)
+        self._keep = []
+
+    @classmethod
+    def get(cls, device=0):
+        if device not in cls._by_device:
+            cls._by_device[device] = cls(device)
+        return cls._by_device[device]
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib.nce_ctx_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def ce_model_create(self, info):
+        """upload the model of a CE cell (dict from ce_gpu_implicit.build_ce_gpu_implicit)."""
+        tpl = info['template']
+        arrs = dict(
+            entries_per_basis=np.ascontiguousarray(tpl.J_b, np.int32),
+            window_per_basis=np.ascontiguousarray(tpl.W_b, np.int32),
+            entry_slot=np.ascontiguousarray(tpl.ent_slot, np.uint16),
+            entry_coef=np.ascontiguousarray(tpl.ent_coef, np.float64),
+            window=np.ascontiguousarray(info['win'], np.int32),
+            rows=np.ascontiguousarray(np.concatenate([tpl.T_b[..., None], info['T_uo']], -1), np.int32),
+            row_bf=np.ascontiguousarray(tpl.T_bf, np.int32),
+            row_coef=np.ascontiguousarray(tpl.T_coef, np.float64),
+            pf=np.ascontiguousarray(tpl.pf, np.float64),
+            basis_to_sl=np.ascontiguousarray(tpl.basis_to_sl, np.int32),
+            active_basis=np.ascontiguousarray(tpl.act_b, np.int32),
+            active_nbasis=np.ascontiguousarray(tpl.act_nb, np.int32),
+            active_cum=np.ascontiguousarray(np.cumsum(tpl.act_p), np.float64),
+        )
+        d = ModelDesc()
+        d.n_basis = tpl.nB; d.max_order = tpl.mo; d.n_entries = tpl.J; d.max_window = tpl.Wmax
+        d.n_rows = tpl.Rt; d.n_sublattices = tpl.pf.shape[0]; d.n_active = len(tpl.act_nb)
+        d.max_active_basis = tpl.act_b.shape[1]
+        for k in range(3):
+            d.d[k] = int(info['d'][k])
+        d.n_cells = int(info['n_cells']); d.factor = float(info['factor']); d.zerolet = float(info['zerolet'])
+        for k, a in arrs.items():
+            setattr(d, k, a.ctypes.data)
+        mid = C.c_int32()
+        check(lib.nce_ce_model_create(self.h, C.byref(d), C.byref(mid)))
+        return int(mid.value)
+
+    def ce_energy(self, model, occ, energy, stream=None):
+        R, N = occ.shape
+        check(lib.nce_ce_energy(self.h, model, ptr(occ), R, N, ptr(energy), stream_ptr(stream)))
+
+    def ce_delta(self, model, occ, ia, ib, dE, precision=PREC_FP64, stream=None):
+        R, N = occ.shape
+        check(lib.nce_ce_delta(self.h, model, ptr(occ), R, N, ptr(ia), ptr(ib), ptr(dE), int(precision),
+                               stream_ptr(stream)))
+
+    def pt_run_chunk(self, jobs, n_steps, precision=PREC_FP64, tile=0, stream=None):
+        arr = (PTJob * len(jobs))(*jobs)
+        check(lib.nce_pt_run_chunk(self.h, C.addressof(arr), len(jobs), int(n_steps), int(precision), int(tile),
+                                   stream_ptr(stream)))
+
+    def pt_exchange(self, jobs, stream=None):
+        arr = (ExchangeJob * len(jobs))(*jobs)
+        check(lib.nce_pt_exchange(self.h, C.addressof(arr), len(jobs), stream_ptr(stream)))

@@ -1,0 +1,11 @@
+#!/bin/bash
+# Build libneuralce_hip.so for gfx950 (in-tree, next to the package).
+set -euo pipefail
+here="$(cd "$(dirname "$0")" && pwd)"
+out="${here}/../libneuralce_hip.so"
+/opt/rocm/bin/hipcc --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
+    -Xclang -target-feature -Xclang -packed-fp32-ops \
+    -Wall -Wno-unused-function \
+    -o "${out}.tmp" "${here}/neuralce_hip.hip"
+mv "${out}.tmp" "${out}"
+echo "built ${out}"

@@ -1,0 +1,885 @@
+// libneuralce_hip.so — gfx950 kernels and C ABI for neuralce's parallel-tempering canonical-swap Monte Carlo.
+//
+// Kernels (one translation unit):
+//   ce_chunk_kernel   n_steps swap-Metropolis steps of one replica per workgroup (persistent over the chunk).
+//                     Per step: Philox proposal, gather the two covering windows (int8 occupations, integer
+//                     arithmetic on the Smith-normal-form cell coordinate), build per-slot point-function value
+//                     tables (old / new) in LDS, evaluate every reverse entry of both sites from the value tables,
+//                     reduce with DPP row reductions + LDS across waves, Metropolis test, two-byte write.
+//   ce_energy_kernel  full energies (forward rows over every cell), fixed-order two-pass reduction.
+//   ce_delta_kernel   swap energy change of explicit proposals (same window path as the chunk kernel).
+//   pt_exchange_kernel temperature-swap replica exchange of each job.
+// Reference functions replaced: see include/neuralce_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/neuralce_hip.h"
+
+// ───────────────────────────────────────────── errors ─────────────────────────────────────────────
+static thread_local std::string g_err;
+
+static int set_err(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(x)                                                                                    \
+    do {                                                                                              \
+        hipError_t e_ = (x);                                                                          \
+        if (e_ != hipSuccess)                                                                         \
+            return set_err(NCE_ERR_HIP, std::string(#x " failed: ") + hipGetErrorString(e_));         \
+    } while (0)
+
+extern "C" const char *nce_last_error(void) { return g_err.c_str(); }
+
+// ───────────────────────────────────────────── Philox ─────────────────────────────────────────────
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11).
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+
+extern "C" void nce_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+// ───────────────────────────────────────────── model ──────────────────────────────────────────────
+struct ModelDev {
+    int32_t nB, J, Wmax, Wcap, mo, Rt, n_act, max_act_b;
+    int32_t d0, d1, d2, n_sl;
+    int64_t n_cells, n_sites;
+    double factor, zerolet;
+    const int32_t *J_b, *W_b;
+    const uint2 *ent_slot;      // [nB*J] four u16 slots, padding -> Wcap-1
+    const double *ent_coef;     // [nB*J]
+    const float *ent_coef_f;    // [nB*J]
+    const int4 *win;            // [nB*Wmax] (basis, k0, k1, k2)
+    const double *pf;           // [n_sl*4]
+    const int32_t *basis_to_sl; // [nB]
+    const int32_t *act_b, *act_nb;
+    const double *act_cum;
+    const int4 *rows;           // [Rt*mo]
+    const int32_t *row_bf;      // [Rt*mo]
+    const double *row_coef;     // [Rt]
+};
+
+struct nce_ctx {
+    int device = 0;
+    int n_cu = 0;
+    size_t max_lds = 0;
+    std::vector<ModelDev> models;
+    std::vector<size_t> lds_chunk64, lds_chunk32, lds_energy;
+    ModelDev *d_models = nullptr;
+    int cap_models = 0;
+    std::vector<void *> allocs;
+    struct Stage {
+        void *host = nullptr, *dev = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+    Stage stage[4];
+    int next_stage = 0;
+    double *d_partial = nullptr;
+    size_t partial_cap = 0;
+};
+
+__host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// LDS layout of the chunk / delta kernels (identical carve on host and device)
+struct ChunkLds {
+    size_t slot, coef, win, valA, valB, red, total;
+};
+
+template <typename Real>
+__host__ __device__ inline ChunkLds chunk_lds_layout(int nB, int J, int Wmax, int Wcap, int waves) {
+    ChunkLds L;
+    size_t o = 0;
+    L.slot = o; o = align16(o + (size_t)nB * J * sizeof(uint2));
+    L.coef = o; o = align16(o + (size_t)nB * J * sizeof(Real));
+    L.win = o;  o = align16(o + (size_t)nB * Wmax * sizeof(int4));
+    L.valA = o; o = align16(o + (size_t)Wcap * 2 * sizeof(Real));
+    L.valB = o; o = align16(o + (size_t)Wcap * 2 * sizeof(Real));
+    L.red = o;  o = align16(o + (size_t)waves * sizeof(double) * 2);
+    L.total = o;
+    return L;
+}
+
+// ───────────────────────────────────────────── reductions ─────────────────────────────────────────
+// Row (16-lane) butterfly with DPP: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror.
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const int64_t b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+template <int CTRL>
+__device__ inline float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ inline double readlane_f64(double v, int lane) {
+    const int64_t b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+
+// wave sum, result uniform in every lane; fixed order (deterministic)
+__device__ inline double wave_sum(double v) {
+    v += dpp_f64<0xB1>(v);
+    v += dpp_f64<0x4E>(v);
+    v += dpp_f64<0x141>(v);
+    v += dpp_f64<0x140>(v);
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+__device__ inline float wave_sum(float v) {
+    v += dpp_f32<0xB1>(v);
+    v += dpp_f32<0x4E>(v);
+    v += dpp_f32<0x141>(v);
+    v += dpp_f32<0x140>(v);
+    const int i = __float_as_int(v);
+    return (__int_as_float(__builtin_amdgcn_readlane(i, 0)) + __int_as_float(__builtin_amdgcn_readlane(i, 16))) +
+           (__int_as_float(__builtin_amdgcn_readlane(i, 32)) + __int_as_float(__builtin_amdgcn_readlane(i, 48)));
+}
+
+template <typename Real, int WAVES>
+__device__ inline Real block_sum(Real v, Real *red) {
+    v = wave_sum(v);
+    if constexpr (WAVES == 1) {
+        return v;
+    } else {
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) red[w] = v;
+        __syncthreads();
+        Real s = red[0];
+#pragma unroll
+        for (int i = 1; i < WAVES; ++i) s += red[i];
+        return s;
+    }
+}
+
+// ───────────────────────────────────────────── swap energy change ─────────────────────────────────
+struct Coord {
+    int32_t b, k0, k1, k2;
+};
+
+__device__ inline Coord site_coord(const ModelDev &m, int64_t i) {
+    Coord c;
+    c.b = (int32_t)(i % m.nB);
+    const int64_t cell = i / m.nB;
+    const int64_t d12 = (int64_t)m.d1 * m.d2;
+    c.k0 = (int32_t)(cell / d12);
+    const int64_t rem = cell - (int64_t)c.k0 * d12;
+    c.k1 = (int32_t)(rem / m.d2);
+    c.k2 = (int32_t)(rem - (int64_t)c.k1 * m.d2);
+    return c;
+}
+
+__device__ inline int64_t window_site(const ModelDev &m, const Coord &c, int4 e) {
+    int32_t k0 = c.k0 + e.y, k1 = c.k1 + e.z, k2 = c.k2 + e.w;
+    k0 -= (k0 >= m.d0) ? m.d0 : 0;
+    k1 -= (k1 >= m.d1) ? m.d1 : 0;
+    k2 -= (k2 >= m.d2) ? m.d2 : 0;
+    return (((int64_t)k0 * m.d1 + k1) * m.d2 + k2) * m.nB + e.x;
+}
+
+template <typename Real>
+struct Real2 {
+    Real o, n;
+};
+
+// Fill the value tables of both windows for the swap (ia: sa -> sb, ib: sb -> sa).
+//   window A (entries of ia): slot value (pf(old), pf(new)) with both overrides applied by site index;
+//   window B (entries of ib): same, except slots that are the site ia get (0, 0), so every entry of ib that also
+//   contains ia contributes nothing (the reference skips those entries, ce_gpu_implicit.py:189-191).
+template <typename Real, int BS>
+__device__ inline void fill_values(const ModelDev &m, const int4 *s_win, Real2<Real> *valA, Real2<Real> *valB,
+                                   const int8_t *__restrict__ occ, int64_t ia, int64_t ib, int sa, int sb) {
+    const Coord ca = site_coord(m, ia), cb = site_coord(m, ib);
+    const int Wa = m.W_b[ca.b], Wb = m.W_b[cb.b];
+    for (int w = threadIdx.x; w < Wa + Wb; w += BS) {
+        const bool isA = w < Wa;
+        const int slot = isA ? w : w - Wa;
+        const Coord &c = isA ? ca : cb;
+        const int4 e = s_win[c.b * m.Wmax + slot];
+        const int64_t s = window_site(m, c, e);
+        const int sp = occ[s];
+        int spn = sp;
+        if (s == ia) spn = sb;
+        if (s == ib) spn = sa;
+        const double *pf = m.pf + m.basis_to_sl[e.x] * 4 + 2;       // bf = 1
+        Real2<Real> v;
+        v.o = (Real)pf[sp];
+        v.n = (Real)pf[spn];
+        if (!isA && s == ia) { v.o = (Real)0; v.n = (Real)0; }
+        (isA ? valA : valB)[slot] = v;
+    }
+}
+
+template <typename Real>
+__device__ inline Real entry_delta(const Real2<Real> *val, uint2 e) {
+    const Real2<Real> a = val[e.x & 0xffff], b = val[e.x >> 16], c = val[e.y & 0xffff], d = val[e.y >> 16];
+    return (a.n * b.n) * (c.n * d.n) - (a.o * b.o) * (c.o * d.o);
+}
+
+// Partial sum of coef * (prod_new - prod_old) over both sites' entries for this thread.
+template <typename Real, int BS>
+__device__ inline Real entries_partial(const ModelDev &m, const uint2 *s_slot, const Real *s_coef,
+                                       const Real2<Real> *valA, const Real2<Real> *valB, int ba, int bb) {
+    Real acc = (Real)0;
+    if (ba == bb) {
+        const int Jb = m.J_b[ba];
+        const uint2 *sl = s_slot + (size_t)ba * m.J;
+        const Real *cf = s_coef + (size_t)ba * m.J;
+        for (int j = threadIdx.x; j < Jb; j += BS) {
+            const uint2 e = sl[j];
+            acc += cf[j] * (entry_delta(valA, e) + entry_delta(valB, e));
+        }
+    } else {
+        for (int pass = 0; pass < 2; ++pass) {
+            const int b = pass ? bb : ba;
+            const Real2<Real> *val = pass ? valB : valA;
+            const int Jb = m.J_b[b];
+            const uint2 *sl = s_slot + (size_t)b * m.J;
+            const Real *cf = s_coef + (size_t)b * m.J;
+            for (int j = threadIdx.x; j < Jb; j += BS) acc += cf[j] * entry_delta(val, sl[j]);
+        }
+    }
+    return acc;
+}
+
+template <typename Real, int BS>
+__device__ inline void load_model_lds(const ModelDev &m, uint2 *s_slot, Real *s_coef, int4 *s_win,
+                                      Real2<Real> *valA, Real2<Real> *valB) {
+    const int nJ = m.nB * m.J;
+    for (int i = threadIdx.x; i < nJ; i += BS) {
+        s_slot[i] = m.ent_slot[i];
+        if constexpr (sizeof(Real) == 8) s_coef[i] = (Real)m.ent_coef[i];
+        else s_coef[i] = (Real)m.ent_coef_f[i];
+    }
+    for (int i = threadIdx.x; i < m.nB * m.Wmax; i += BS) s_win[i] = m.win[i];
+    if (threadIdx.x == 0) {                         // the constant slot: point-function value 1 before and after
+        valA[m.Wcap - 1].o = valA[m.Wcap - 1].n = (Real)1;
+        valB[m.Wcap - 1].o = valB[m.Wcap - 1].n = (Real)1;
+    }
+}
+
+// ───────────────────────────────────────────── proposal ───────────────────────────────────────────
+struct Proposal {
+    int64_t ia, ib;
+    double u;
+};
+
+__device__ inline uint64_t umulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+// Stream layout shared with the oracle: x = philox({step, replica, 0}), y = philox({step, replica, 1}).
+__device__ inline Proposal draw_proposal(const ModelDev &m, uint64_t seed, uint64_t step, uint32_t rep) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint32_t x[4] = {(uint32_t)step, (uint32_t)(step >> 32), rep, 0u};
+    uint32_t y[4] = {(uint32_t)step, (uint32_t)(step >> 32), rep, 1u};
+    philox4x32_10(x, k0, k1);
+    philox4x32_10(y, k0, k1);
+    const uint64_t ca = umulhi64(((uint64_t)x[1] << 32) | x[0], (uint64_t)m.n_cells);
+    const uint64_t cb = umulhi64(((uint64_t)x[3] << 32) | x[2], (uint64_t)m.n_cells);
+    int sl = 0;
+    if (m.n_act > 1) {
+        const double us = (double)y[2] * (1.0 / 4294967296.0);
+        while (sl < m.n_act - 1 && !(us < m.act_cum[sl])) ++sl;
+    }
+    const uint32_t nb = (uint32_t)m.act_nb[sl];
+    const int ba = m.act_b[sl * m.max_act_b + (int)(((y[3] & 0xffffu) * nb) >> 16)];
+    const int bb = m.act_b[sl * m.max_act_b + (int)(((y[3] >> 16) * nb) >> 16)];
+    Proposal p;
+    p.ia = (int64_t)ca * m.nB + ba;
+    p.ib = (int64_t)cb * m.nB + bb;
+    p.u = ((double)((((uint64_t)y[1] << 32) | y[0]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    return p;
+}
+
+// ───────────────────────────────────────────── chunk kernel ───────────────────────────────────────
+template <typename Real, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ce_chunk_kernel(const ModelDev *__restrict__ models,
+                                                              const nce_pt_job *__restrict__ jobs,
+                                                              const int2 *__restrict__ items, int64_t n_steps) {
+    constexpr int BS = 64 * WAVES;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int2 it = items[blockIdx.x];
+    const nce_pt_job job = jobs[it.x];
+    const ModelDev m = models[job.model];
+    const int r = it.y;
+    const ChunkLds L = chunk_lds_layout<Real>(m.nB, m.J, m.Wmax, m.Wcap, WAVES);
+    uint2 *s_slot = (uint2 *)(smem + L.slot);
+    Real *s_coef = (Real *)(smem + L.coef);
+    int4 *s_win = (int4 *)(smem + L.win);
+    Real2<Real> *valA = (Real2<Real> *)(smem + L.valA);
+    Real2<Real> *valB = (Real2<Real> *)(smem + L.valB);
+    Real *red = (Real *)(smem + L.red);
+    load_model_lds<Real, BS>(m, s_slot, s_coef, s_win, valA, valB);
+    __syncthreads();
+
+    int8_t *occ = job.occ + (int64_t)r * job.n_sites;
+    double E = job.energy[r];
+    const double beta = job.beta[r];
+    const float beta_f = (float)beta;
+    const int slot = job.slot_of_rep[r];
+    const uint64_t step0 = (uint64_t)job.step[r];
+    double emin = job.e_min[r];
+    const double eref = job.accumulate ? job.e_ref[r] : 0.0;
+    double s1 = 0.0, s2 = 0.0;
+    int64_t nacc = 0;
+    const int64_t inj_base = (int64_t)r * n_steps;
+
+    for (int64_t t = 0; t < n_steps; ++t) {
+        Proposal p;
+        if (job.inj_ia) {
+            p.ia = job.inj_ia[inj_base + t];
+            p.ib = job.inj_ib[inj_base + t];
+            p.u = job.inj_u[inj_base + t];
+        } else {
+            p = draw_proposal(m, job.seed, step0 + (uint64_t)t, (uint32_t)r);
+        }
+        const int sa = occ[p.ia], sb = occ[p.ib];
+        if (sa != sb) {
+            fill_values<Real, BS>(m, s_win, valA, valB, occ, p.ia, p.ib, sa, sb);
+            __syncthreads();
+            const int ba = (int)(p.ia % m.nB), bb = (int)(p.ib % m.nB);
+            Real part = entries_partial<Real, BS>(m, s_slot, s_coef, valA, valB, ba, bb);
+            const Real sum = block_sum<Real, WAVES>(part, red);
+            bool accept;
+            double dE;
+            if constexpr (sizeof(Real) == 8) {
+                dE = (double)sum * m.factor;
+                accept = log(p.u) < -beta * dE;
+            } else {
+                const float dEf = (float)sum * (float)m.factor;
+                dE = (double)dEf;
+                accept = logf((float)p.u) < -(beta_f * dEf);
+            }
+            if (accept) {
+                if (threadIdx.x == 0) {
+                    occ[p.ia] = (int8_t)sb;
+                    occ[p.ib] = (int8_t)sa;
+                }
+                E += dE;
+                ++nacc;
+            }
+        }
+        emin = fmin(emin, E);
+        if (job.accumulate) {
+            const double x = E - eref;
+            s1 += x;
+            s2 += x * x;
+        }
+        if (job.traj && ((t + 1) % job.sample_interval) == 0 && threadIdx.x == 0)
+            job.traj[((t + 1) / job.sample_interval - 1) * job.n_replicas + slot] = E;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        job.energy[r] = E;
+        job.e_min[r] = emin;
+        job.n_accept[r] += nacc;
+        job.step[r] = (int64_t)(step0 + (uint64_t)n_steps);
+        if (job.accumulate) {
+            job.s1[r] += s1;
+            job.s2[r] += s2;
+        }
+    }
+}
+
+// ───────────────────────────────────────────── single delta kernel ────────────────────────────────
+template <typename Real, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void ce_delta_kernel(const ModelDev *__restrict__ models, int model,
+                                                              const int8_t *__restrict__ occ_all, int64_t N,
+                                                              const int64_t *__restrict__ ia_all,
+                                                              const int64_t *__restrict__ ib_all, double *dE_out) {
+    constexpr int BS = 64 * WAVES;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ModelDev m = models[model];
+    const ChunkLds L = chunk_lds_layout<Real>(m.nB, m.J, m.Wmax, m.Wcap, WAVES);
+    uint2 *s_slot = (uint2 *)(smem + L.slot);
+    Real *s_coef = (Real *)(smem + L.coef);
+    int4 *s_win = (int4 *)(smem + L.win);
+    Real2<Real> *valA = (Real2<Real> *)(smem + L.valA);
+    Real2<Real> *valB = (Real2<Real> *)(smem + L.valB);
+    Real *red = (Real *)(smem + L.red);
+    load_model_lds<Real, BS>(m, s_slot, s_coef, s_win, valA, valB);
+    __syncthreads();
+    const int r = blockIdx.x;
+    const int8_t *occ = occ_all + (int64_t)r * N;
+    const int64_t ia = ia_all[r], ib = ib_all[r];
+    const int sa = occ[ia], sb = occ[ib];
+    fill_values<Real, BS>(m, s_win, valA, valB, occ, ia, ib, sa, sb);
+    __syncthreads();
+    const Real part = entries_partial<Real, BS>(m, s_slot, s_coef, valA, valB, (int)(ia % m.nB), (int)(ib % m.nB));
+    const Real sum = block_sum<Real, WAVES>(part, red);
+    if (threadIdx.x == 0) {
+        if constexpr (sizeof(Real) == 8) dE_out[r] = (double)sum * m.factor;
+        else dE_out[r] = (double)((float)sum * (float)m.factor);
+    }
+}
+
+// ───────────────────────────────────────────── energy kernels ─────────────────────────────────────
+constexpr int ENERGY_BS = 256;
+
+__global__ __launch_bounds__(ENERGY_BS) void ce_energy_kernel(const ModelDev *__restrict__ models, int model,
+                                                              const int8_t *__restrict__ occ_all, int64_t N,
+                                                              double *__restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ModelDev m = models[model];
+    const int nrm = m.Rt * m.mo;
+    int4 *s_rows = (int4 *)smem;                        // x = basis | (sublattice * 2 + bf) << 16
+    double *s_coef = (double *)(smem + align16((size_t)nrm * sizeof(int4)));
+    double *s_pf = s_coef + m.Rt;                      // [n_sl][2][2]
+    double *s_red = s_pf + m.n_sl * 4;
+    for (int i = threadIdx.x; i < nrm; i += ENERGY_BS) {
+        int4 e = m.rows[i];
+        e.x = e.x | ((m.basis_to_sl[e.x] * 2 + m.row_bf[i]) << 16);
+        s_rows[i] = e;
+    }
+    for (int i = threadIdx.x; i < m.Rt; i += ENERGY_BS) s_coef[i] = m.row_coef[i];
+    for (int i = threadIdx.x; i < m.n_sl * 4; i += ENERGY_BS) s_pf[i] = m.pf[i];
+    __syncthreads();
+    const int r = blockIdx.y;
+    const int8_t *occ = occ_all + (int64_t)r * N;
+    const int64_t cell = (int64_t)blockIdx.x * ENERGY_BS + threadIdx.x;
+    double acc = 0.0;
+    if (cell < m.n_cells) {
+        const Coord c = site_coord(m, cell * m.nB);
+        for (int row = 0; row < m.Rt; ++row) {
+            double p = 1.0;
+            for (int q = 0; q < m.mo; ++q) {
+                int4 e = s_rows[row * m.mo + q];
+                const int pfo = (e.x >> 16) * 2;
+                e.x &= 0xffff;
+                const int64_t s = window_site(m, c, e);
+                p *= s_pf[pfo + occ[s]];
+            }
+            acc += s_coef[row] * p;
+        }
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < ENERGY_BS / 64; ++w) s += s_red[w];
+        partial[(int64_t)r * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+__global__ void ce_energy_finish_kernel(const ModelDev *__restrict__ models, int model,
+                                        const double *__restrict__ partial, int n_part, double *E) {
+    const ModelDev m = models[model];
+    const int r = blockIdx.x;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n_part; i += 64) acc += partial[(int64_t)r * n_part + i];
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) E[r] = m.zerolet + m.factor * acc;
+}
+
+// ───────────────────────────────────────────── exchange kernel ────────────────────────────────────
+__global__ void pt_exchange_kernel(const nce_pt_exchange_job *__restrict__ jobs) {
+    const nce_pt_exchange_job j = jobs[blockIdx.x];
+    if (threadIdx.x != 0) return;
+    const int R = j.n_replicas;
+    const uint32_t k0 = (uint32_t)j.seed, k1 = (uint32_t)(j.seed >> 32);
+    int parity;
+    if (j.inj_parity >= 0) {
+        parity = j.inj_parity & 1;
+    } else {
+        uint32_t z[4] = {(uint32_t)j.ex_index, (uint32_t)(j.ex_index >> 32), 0xffffffffu, 2u};
+        philox4x32_10(z, k0, k1);
+        parity = (int)(z[0] & 1u);
+    }
+    int64_t ns = 0;
+    for (int p = 0; 2 * p <= R - 2; ++p) {
+        const int a = 2 * p + parity, b = a + 1;
+        if (b >= R) continue;
+        if (j.ladder_id && j.ladder_id[a] != j.ladder_id[b]) continue;
+        double u;
+        if (j.inj_u) {
+            u = j.inj_u[p];
+        } else {
+            uint32_t w[4] = {(uint32_t)j.ex_index, (uint32_t)(j.ex_index >> 32), (uint32_t)p, 3u};
+            philox4x32_10(w, k0, k1);
+            u = ((double)((((uint64_t)w[1] << 32) | w[0]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        }
+        const double la = (j.beta_slot[b] - j.beta_slot[a]) * (j.energy[j.rep_of_slot[b]] - j.energy[j.rep_of_slot[a]]);
+        if (log(u) < la) {
+            const int t = j.rep_of_slot[a];
+            j.rep_of_slot[a] = j.rep_of_slot[b];
+            j.rep_of_slot[b] = t;
+            ++ns;
+        }
+    }
+    for (int t = 0; t < R; ++t) {
+        const int rep = j.rep_of_slot[t];
+        j.beta_rep[rep] = j.beta_slot[t];
+        j.slot_of_rep[rep] = t;
+        if (j.traj_last) j.traj_last[t] = j.energy[rep];
+    }
+    j.n_swaps[0] += ns;
+}
+
+// ───────────────────────────────────────────── host side ──────────────────────────────────────────
+extern "C" int nce_ctx_create(int32_t device_ordinal, nce_ctx **out) {
+    if (!out) return set_err(NCE_ERR_ARG, "out is NULL");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device_ordinal < 0 || device_ordinal >= n)
+        return set_err(NCE_ERR_ARG, "device ordinal " + std::to_string(device_ordinal) + " out of range (" +
+                                        std::to_string(n) + " devices)");
+    HIP_TRY(hipSetDevice(device_ordinal));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device_ordinal));
+    nce_ctx *c = new nce_ctx();
+    c->device = device_ordinal;
+    c->n_cu = prop.multiProcessorCount;
+    c->max_lds = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
+    for (auto &s : c->stage) {
+        if (hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess) {
+            delete c;
+            return set_err(NCE_ERR_HIP, "hipEventCreate failed");
+        }
+    }
+    *out = c;
+    return NCE_OK;
+}
+
+extern "C" void nce_ctx_destroy(nce_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->d_models) (void)hipFree(c->d_models);
+    if (c->d_partial) (void)hipFree(c->d_partial);
+    for (auto &s : c->stage) {
+        if (s.host) (void)hipHostFree(s.host);
+        if (s.dev) (void)hipFree(s.dev);
+        if (s.ev) (void)hipEventDestroy(s.ev);
+    }
+    delete c;
+}
+
+template <typename T>
+static int upload(nce_ctx *c, const T *host, size_t n, const T **dev) {
+    if (n == 0) { *dev = nullptr; return NCE_OK; }
+    if (!host) return set_err(NCE_ERR_ARG, "missing host array");
+    void *p = nullptr;
+    HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+    c->allocs.push_back(p);
+    HIP_TRY(hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice));
+    *dev = (const T *)p;
+    return NCE_OK;
+}
+
+#define UP(h, n, d)                                                                                    \
+    do {                                                                                               \
+        int rc_ = upload(c, h, n, d);                                                                  \
+        if (rc_) return rc_;                                                                           \
+    } while (0)
+
+static int sync_models(nce_ctx *c) {
+    if ((int)c->models.size() > c->cap_models) {
+        int cap = std::max(16, 2 * (int)c->models.size());
+        void *p = nullptr;
+        HIP_TRY(hipMalloc(&p, cap * sizeof(ModelDev)));
+        if (c->d_models) HIP_TRY(hipFree(c->d_models));
+        c->d_models = (ModelDev *)p;
+        c->cap_models = cap;
+    }
+    HIP_TRY(hipMemcpy(c->d_models, c->models.data(), c->models.size() * sizeof(ModelDev), hipMemcpyHostToDevice));
+    return NCE_OK;
+}
+
+extern "C" int nce_ce_model_create(nce_ctx *c, const nce_ce_model_desc *d, int32_t *model_id) {
+    if (!c || !d || !model_id) return set_err(NCE_ERR_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    if (d->n_basis < 1 || d->max_order < 1 || d->max_order > 4 || d->n_entries < 1 || d->max_window < 1 ||
+        d->n_rows < 1 || d->n_active < 1 || d->max_active_basis < 1)
+        return set_err(NCE_ERR_ARG, "model sizes out of range (max_order must be 1..4)");
+    for (int i = 0; i < 3; ++i)
+        if (d->d[i] < 1) return set_err(NCE_ERR_ARG, "Smith normal form diagonal must be positive");
+    if ((int64_t)d->d[0] * d->d[1] * d->d[2] != d->n_cells) return set_err(NCE_ERR_ARG, "n_cells != d0*d1*d2");
+    ModelDev m{};
+    m.nB = d->n_basis; m.J = d->n_entries; m.Wmax = d->max_window; m.mo = d->max_order; m.Rt = d->n_rows;
+    m.n_act = d->n_active; m.max_act_b = d->max_active_basis; m.n_sl = d->n_sublattices;
+    m.d0 = d->d[0]; m.d1 = d->d[1]; m.d2 = d->d[2];
+    m.n_cells = d->n_cells; m.n_sites = d->n_cells * d->n_basis;
+    m.factor = d->factor; m.zerolet = d->zerolet;
+    int Wcap = 1;
+    while (Wcap < m.Wmax + 1) Wcap <<= 1;
+    m.Wcap = Wcap;
+    const size_t nJ = (size_t)m.nB * m.J;
+    // validate and remap slots (0xFFFF -> Wcap - 1)
+    std::vector<uint2> slots(nJ);
+    for (size_t i = 0; i < nJ; ++i) {
+        uint32_t q[4];
+        const int b = (int)(i / m.J);
+        for (int k = 0; k < 4; ++k) {
+            uint32_t s = d->entry_slot[i * 4 + k];
+            if (s == 0xFFFFu) s = (uint32_t)(Wcap - 1);
+            else if ((int)s >= d->window_per_basis[b]) return set_err(NCE_ERR_ARG, "entry slot outside its window");
+            q[k] = s;
+        }
+        slots[i] = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
+    }
+    for (int b = 0; b < m.nB; ++b) {
+        if (d->entries_per_basis[b] < 0 || d->entries_per_basis[b] > m.J || d->window_per_basis[b] < 1 ||
+            d->window_per_basis[b] > m.Wmax)
+            return set_err(NCE_ERR_ARG, "per-basis entry / window counts out of range");
+    }
+    const size_t nW = (size_t)m.nB * m.Wmax, nR = (size_t)m.Rt * m.mo;
+    for (size_t i = 0; i < nW; ++i) {
+        const int32_t *w = d->window + 4 * i;
+        if (w[0] < 0 || w[0] >= m.nB || w[1] < 0 || w[1] >= m.d0 || w[2] < 0 || w[2] >= m.d1 || w[3] < 0 ||
+            w[3] >= m.d2)
+            return set_err(NCE_ERR_ARG, "window offset not reduced into the cell");
+    }
+    for (size_t i = 0; i < nR; ++i) {
+        const int32_t *w = d->rows + 4 * i;
+        if (w[0] < 0 || w[0] >= m.nB || w[1] < 0 || w[1] >= m.d0 || w[2] < 0 || w[2] >= m.d1 || w[3] < 0 ||
+            w[3] >= m.d2 || d->row_bf[i] < 0 || d->row_bf[i] > 1)
+            return set_err(NCE_ERR_ARG, "forward row offset not reduced into the cell");
+    }
+    for (int b = 0; b < m.nB; ++b)
+        if (d->basis_to_sl[b] < 0 || d->basis_to_sl[b] >= m.n_sl) return set_err(NCE_ERR_ARG, "basis_to_sl");
+    for (int a = 0; a < m.n_act; ++a) {
+        if (d->active_nbasis[a] < 1 || d->active_nbasis[a] > m.max_act_b) return set_err(NCE_ERR_ARG, "active_nbasis");
+        for (int k = 0; k < d->active_nbasis[a]; ++k) {
+            const int b = d->active_basis[a * m.max_act_b + k];
+            if (b < 0 || b >= m.nB) return set_err(NCE_ERR_ARG, "active_basis");
+        }
+    }
+    std::vector<float> coef_f(nJ);
+    for (size_t i = 0; i < nJ; ++i) coef_f[i] = (float)d->entry_coef[i];
+    UP(d->entries_per_basis, (size_t)m.nB, &m.J_b);
+    UP(d->window_per_basis, (size_t)m.nB, &m.W_b);
+    UP(slots.data(), nJ, &m.ent_slot);
+    UP(d->entry_coef, nJ, &m.ent_coef);
+    UP(coef_f.data(), nJ, &m.ent_coef_f);
+    UP((const int4 *)d->window, nW, &m.win);
+    UP(d->pf, (size_t)m.n_sl * 4, &m.pf);
+    UP(d->basis_to_sl, (size_t)m.nB, &m.basis_to_sl);
+    UP(d->active_basis, (size_t)m.n_act * m.max_act_b, &m.act_b);
+    UP(d->active_nbasis, (size_t)m.n_act, &m.act_nb);
+    UP(d->active_cum, (size_t)m.n_act, &m.act_cum);
+    UP((const int4 *)d->rows, nR, &m.rows);
+    UP(d->row_bf, nR, &m.row_bf);
+    UP(d->row_coef, (size_t)m.Rt, &m.row_coef);
+    c->models.push_back(m);
+    c->lds_chunk64.push_back(chunk_lds_layout<double>(m.nB, m.J, m.Wmax, m.Wcap, 8).total);
+    c->lds_chunk32.push_back(chunk_lds_layout<float>(m.nB, m.J, m.Wmax, m.Wcap, 8).total);
+    c->lds_energy.push_back(align16(nR * sizeof(int4)) + (m.Rt + m.n_sl * 4 + ENERGY_BS / 64) * sizeof(double));
+    int rc = sync_models(c);
+    if (rc) return rc;
+    *model_id = (int32_t)c->models.size() - 1;
+    return NCE_OK;
+}
+
+static int check_model(nce_ctx *c, int32_t model) {
+    if (!c) return set_err(NCE_ERR_ARG, "NULL context");
+    if (model < 0 || model >= (int32_t)c->models.size()) return set_err(NCE_ERR_ARG, "unknown model id");
+    return NCE_OK;
+}
+
+template <typename K>
+static int allow_lds(K kernel, size_t lds) {
+    if (lds > 65536)
+        HIP_TRY(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    return NCE_OK;
+}
+
+extern "C" int nce_ce_energy(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, double *E,
+                             void *stream) {
+    int rc = check_model(c, model);
+    if (rc) return rc;
+    const ModelDev &m = c->models[model];
+    if (N != m.n_sites) return set_err(NCE_ERR_ARG, "n_sites does not match the model's cell");
+    if (R < 1 || !occ || !E) return set_err(NCE_ERR_ARG, "bad replica count or NULL buffer");
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t nblk64 = (m.n_cells + ENERGY_BS - 1) / ENERGY_BS;
+    if (nblk64 > (1 << 30)) return set_err(NCE_ERR_ARG, "cell too large");
+    const int nblk = (int)nblk64;
+    const size_t need = (size_t)R * nblk;
+    if (need > c->partial_cap) {
+        if (c->d_partial) {
+            HIP_TRY(hipDeviceSynchronize());
+            HIP_TRY(hipFree(c->d_partial));
+        }
+        HIP_TRY(hipMalloc(&c->d_partial, need * sizeof(double)));
+        c->partial_cap = need;
+    }
+    const size_t lds = c->lds_energy[model];
+    if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "forward rows do not fit in LDS");
+    hipStream_t s = (hipStream_t)stream;
+    if (int rc2 = allow_lds(ce_energy_kernel, lds)) return rc2;
+    hipLaunchKernelGGL(ce_energy_kernel, dim3(nblk, R), dim3(ENERGY_BS), lds, s, c->d_models, model, occ, N,
+                       c->d_partial);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(ce_energy_finish_kernel, dim3(R), dim3(64), 0, s, c->d_models, model, c->d_partial, nblk, E);
+    HIP_TRY(hipGetLastError());
+    return NCE_OK;
+}
+
+template <typename Real, int W>
+static int launch_delta(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, const int64_t *ia,
+                        const int64_t *ib, double *dE, hipStream_t s) {
+    const ModelDev &m = c->models[model];
+    const size_t lds = chunk_lds_layout<Real>(m.nB, m.J, m.Wmax, m.Wcap, W).total;
+    if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "template does not fit in LDS");
+    if (int rc = allow_lds(ce_delta_kernel<Real, W>, lds)) return rc;
+    hipLaunchKernelGGL((ce_delta_kernel<Real, W>), dim3(R), dim3(64 * W), lds, s, c->d_models, model, occ, N, ia, ib,
+                       dE);
+    HIP_TRY(hipGetLastError());
+    return NCE_OK;
+}
+
+extern "C" int nce_ce_delta(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, const int64_t *ia,
+                            const int64_t *ib, double *dE, int32_t precision, void *stream) {
+    int rc = check_model(c, model);
+    if (rc) return rc;
+    if (N != c->models[model].n_sites) return set_err(NCE_ERR_ARG, "n_sites does not match the model's cell");
+    if (R < 1 || !occ || !ia || !ib || !dE) return set_err(NCE_ERR_ARG, "bad replica count or NULL buffer");
+    HIP_TRY(hipSetDevice(c->device));
+    if (precision == NCE_PREC_FP64) return launch_delta<double, 4>(c, model, occ, R, N, ia, ib, dE, (hipStream_t)stream);
+    if (precision == NCE_PREC_MIXED) return launch_delta<float, 4>(c, model, occ, R, N, ia, ib, dE, (hipStream_t)stream);
+    return set_err(NCE_ERR_ARG, "unknown precision");
+}
+
+// stage a host buffer into device memory, stream-ordered; a ring of pinned buffers guarded by events
+static int stage_upload(nce_ctx *c, const void *host, size_t bytes, hipStream_t s, void **dev) {
+    nce_ctx::Stage &st = c->stage[c->next_stage];
+    c->next_stage = (c->next_stage + 1) % 4;
+    if (st.pending) {
+        HIP_TRY(hipEventSynchronize(st.ev));
+        st.pending = false;
+    }
+    if (bytes > st.cap) {
+        if (st.host) HIP_TRY(hipHostFree(st.host));
+        if (st.dev) HIP_TRY(hipFree(st.dev));
+        size_t cap = std::max<size_t>(4096, bytes * 2);
+        HIP_TRY(hipHostMalloc(&st.host, cap, hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&st.dev, cap));
+        st.cap = cap;
+    }
+    std::memcpy(st.host, host, bytes);
+    HIP_TRY(hipMemcpyAsync(st.dev, st.host, bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(st.ev, s));
+    st.pending = true;
+    *dev = st.dev;
+    return NCE_OK;
+}
+
+template <typename Real, int W>
+static int launch_chunk(nce_ctx *c, const nce_pt_job *d_jobs, const int2 *d_items, int n_items, int64_t n_steps,
+                        size_t lds, hipStream_t s) {
+    if (int rc = allow_lds(ce_chunk_kernel<Real, W>, lds)) return rc;
+    hipLaunchKernelGGL((ce_chunk_kernel<Real, W>), dim3(n_items), dim3(64 * W), lds, s, c->d_models, d_jobs, d_items,
+                       n_steps);
+    HIP_TRY(hipGetLastError());
+    return NCE_OK;
+}
+
+extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps,
+                                int32_t precision, int32_t tile, void *stream) {
+    if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
+    if (n_steps < 0) return set_err(NCE_ERR_ARG, "negative n_steps");
+    if (precision != NCE_PREC_FP64 && precision != NCE_PREC_MIXED) return set_err(NCE_ERR_ARG, "unknown precision");
+    if (n_steps == 0) return NCE_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<int2> items;
+    size_t lds_need = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        const nce_pt_job &jb = jobs[j];
+        int rc = check_model(c, jb.model);
+        if (rc) return rc;
+        const ModelDev &m = c->models[jb.model];
+        if (jb.n_sites != m.n_sites) return set_err(NCE_ERR_ARG, "job n_sites does not match its model");
+        if (jb.n_replicas < 1) return set_err(NCE_ERR_ARG, "job without replicas");
+        if (!jb.occ || !jb.energy || !jb.beta || !jb.slot_of_rep || !jb.step || !jb.n_accept || !jb.e_min)
+            return set_err(NCE_ERR_ARG, "job has a NULL state buffer");
+        if (jb.traj && (jb.sample_interval < 1 || n_steps % jb.sample_interval != 0))
+            return set_err(NCE_ERR_ARG, "n_steps must be a multiple of sample_interval");
+        if (jb.accumulate && (!jb.e_ref || !jb.s1 || !jb.s2)) return set_err(NCE_ERR_ARG, "accumulators missing");
+        if ((jb.inj_ia != nullptr) != (jb.inj_ib != nullptr) || (jb.inj_ia != nullptr) != (jb.inj_u != nullptr))
+            return set_err(NCE_ERR_ARG, "injected streams must be all set or all NULL");
+        for (int r = 0; r < jb.n_replicas; ++r) items.push_back(make_int2(j, r));
+        lds_need = std::max(lds_need, precision == NCE_PREC_FP64 ? c->lds_chunk64[jb.model] : c->lds_chunk32[jb.model]);
+    }
+    if (lds_need > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "template does not fit in LDS");
+    int W = tile;
+    if (W == 0) {                                      // default tiling: fill the CUs with waves per replica
+        const int per_cu = (int)((items.size() + c->n_cu - 1) / c->n_cu);
+        W = per_cu >= 4 ? 1 : per_cu >= 2 ? 2 : 4;
+    }
+    if (W != 1 && W != 2 && W != 4 && W != 8) return set_err(NCE_ERR_ARG, "tile must be 0, 1, 2, 4 or 8");
+    // job table and work items in one staged buffer
+    const size_t jb_bytes = align16(sizeof(nce_pt_job) * n_jobs);
+    std::vector<unsigned char> buf(jb_bytes + items.size() * sizeof(int2));
+    std::memcpy(buf.data(), jobs, sizeof(nce_pt_job) * n_jobs);
+    std::memcpy(buf.data() + jb_bytes, items.data(), items.size() * sizeof(int2));
+    hipStream_t s = (hipStream_t)stream;
+    void *dev = nullptr;
+    int rc = stage_upload(c, buf.data(), buf.size(), s, &dev);
+    if (rc) return rc;
+    const nce_pt_job *d_jobs = (const nce_pt_job *)dev;
+    const int2 *d_items = (const int2 *)((unsigned char *)dev + jb_bytes);
+    const int n_items = (int)items.size();
+    // recompute the LDS bytes for the chosen wave count (reduction scratch depends on it)
+    size_t lds = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        const ModelDev &m = c->models[jobs[j].model];
+        lds = std::max(lds, precision == NCE_PREC_FP64 ? chunk_lds_layout<double>(m.nB, m.J, m.Wmax, m.Wcap, W).total
+                                                       : chunk_lds_layout<float>(m.nB, m.J, m.Wmax, m.Wcap, W).total);
+    }
+    if (precision == NCE_PREC_FP64) {
+        switch (W) {
+            case 1: return launch_chunk<double, 1>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+            case 2: return launch_chunk<double, 2>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+            case 4: return launch_chunk<double, 4>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+            default: return launch_chunk<double, 8>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+        }
+    }
+    switch (W) {
+        case 1: return launch_chunk<float, 1>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+        case 2: return launch_chunk<float, 2>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+        case 4: return launch_chunk<float, 4>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+        default: return launch_chunk<float, 8>(c, d_jobs, d_items, n_items, n_steps, lds, s);
+    }
+}
+
+extern "C" int nce_pt_exchange(nce_ctx *c, const nce_pt_exchange_job *jobs, int32_t n_jobs, void *stream) {
+    if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
+    for (int j = 0; j < n_jobs; ++j) {
+        const nce_pt_exchange_job &e = jobs[j];
+        if (e.n_replicas < 1 || !e.energy || !e.rep_of_slot || !e.slot_of_rep || !e.beta_slot || !e.beta_rep ||
+            !e.n_swaps)
+            return set_err(NCE_ERR_ARG, "exchange job has a NULL buffer");
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    void *dev = nullptr;
+    int rc = stage_upload(c, jobs, sizeof(nce_pt_exchange_job) * n_jobs, s, &dev);
+    if (rc) return rc;
+    hipLaunchKernelGGL(pt_exchange_kernel, dim3(n_jobs), dim3(64), 0, s, (const nce_pt_exchange_job *)dev);
+    HIP_TRY(hipGetLastError());
+    return NCE_OK;
+}

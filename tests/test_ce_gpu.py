@@ -1,0 +1,167 @@
+"""GPU parity of the cluster-expansion kernels against the FP64 oracle (calls go through libneuralce_hip.so).
+
+Bars: occupations bit-exact; FP64 energies and swap energy changes within 1e-9 eV; mixed precision (FP32 swap energy
+change) within 1e-5 eV per swap; Metropolis decisions identical except where |beta dE + ln u| < 1e-6."""
+import numpy as np
+import pytest
+
+from tests.conftest import random_fcc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def _setup(ce, tpl, n, x=0.5, seed=1, scale=1.0, prim_reps=None):
+    from neuralce2_amd.ce.ce_gpu_implicit import build_ce_gpu_implicit, encode_canonical
+    from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
+    from oracle.ce_oracle import OracleCE
+    if prim_reps is not None:
+        from neuralce2_amd.lattice import bulk_fcc
+        at = bulk_fcc("Ni", 3.924, cubic=False).repeat(prim_reps)
+        r = np.random.RandomState(seed); Z = np.full(len(at), 28); Z[r.choice(len(at), len(at) // 2, replace=False)] = 78
+        at.numbers = Z
+    else:
+        at = random_fcc(n, x=x, seed=seed)
+    g = GPUClusterExpansion(ce, at, energy_scale=scale)
+    o = OracleCE(g.info)
+    sig = g.encode(at).astype(np.int8)
+    return g, o, sig
+
+
+CELLS = [dict(n=4), dict(n=8), dict(n=1), dict(n=2), dict(prim_reps=(2, 2, 2)), dict(prim_reps=(3, 1, 2))]
+
+
+@pytest.mark.parametrize("cell", CELLS, ids=lambda c: str(c))
+def test_energy_matches_oracle(torch_dev, ptni_ce, ptni_template, cell):
+    import torch
+    g, o, sig = _setup(ptni_ce, ptni_template, cell.get("n", 0), prim_reps=cell.get("prim_reps"))
+    rng = np.random.default_rng(3)
+    R = 5
+    batch = np.stack([rng.permutation(sig) for _ in range(R)])
+    E_gpu = g.batched_energy(batch)
+    E_ref = np.array([o.energy(b) for b in batch])
+    assert np.max(np.abs(E_gpu - E_ref)) < 1e-9, (E_gpu, E_ref)
+
+
+@pytest.mark.parametrize("cell", CELLS, ids=lambda c: str(c))
+def test_swap_delta_matches_oracle(torch_dev, ptni_ce, ptni_template, cell):
+    import torch
+    from neuralce2_amd._lib import PREC_FP64, PREC_MIXED
+    g, o, sig = _setup(ptni_ce, ptni_template, cell.get("n", 0), prim_reps=cell.get("prim_reps"))
+    N = len(sig)
+    rng = np.random.default_rng(7)
+    R = 64
+    occ = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
+    ia = rng.integers(0, N, R); ib = rng.integers(0, N, R)
+    # near pairs: b inside a's window (clusters containing both sites)
+    nb = g.info['win'][0, :, 1:]
+    for k in range(0, R, 4):
+        ib[k] = (ia[k] + rng.integers(1, 4)) % N
+    ib[1] = ia[1]                                   # same site
+    ref = np.array([o.delta(occ[r], ia[r], ib[r]) for r in range(R)])
+    t_occ = torch.tensor(occ, device=torch_dev)
+    t_ia = torch.tensor(ia, dtype=torch.int64, device=torch_dev); t_ib = torch.tensor(ib, dtype=torch.int64, device=torch_dev)
+    dE = torch.empty(R, dtype=torch.float64, device=torch_dev)
+    g.ctx.ce_delta(g.model, t_occ, t_ia, t_ib, dE, PREC_FP64)
+    got = dE.cpu().numpy()
+    assert np.max(np.abs(got - ref)) < 1e-9, np.max(np.abs(got - ref))
+    g.ctx.ce_delta(g.model, t_occ, t_ia, t_ib, dE, PREC_MIXED)
+    got32 = dE.cpu().numpy()
+    assert np.max(np.abs(got32 - ref)) < 1e-5 * max(1.0, np.abs(ref).max()), np.max(np.abs(got32 - ref))
+
+
+def _near_tie(o, occ_before_path, *a):
+    return False
+
+
+@pytest.mark.parametrize("n,R,steps,precision", [(4, 8, 400, 0), (8, 32, 300, 0), (2, 4, 500, 0), (4, 8, 400, 1)])
+def test_chunk_injected_streams_bitexact(torch_dev, ptni_ce, ptni_template, n, R, steps, precision):
+    """same injected proposals and uniforms -> identical occupations; energies within tolerance."""
+    import torch
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
+    g, o, sig = _setup(ptni_ce, ptni_template, n)
+    N = len(sig)
+    rng = np.random.default_rng(11)
+    occ0 = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
+    betas = 1.0 / (8.617333e-5 * np.geomspace(300, 1400, R).astype(np.float32))
+    betas = betas.astype(np.float32).astype(np.float64)
+    si = 10
+    st = PTJobState(g, occ0, betas, seed=5, sample_interval=si, samples_per_chunk=steps // si)
+    ia = rng.integers(0, N, (R, steps)); ib = rng.integers(0, N, (R, steps)); u = rng.random((R, steps))
+    inj = (torch.tensor(ia, dtype=torch.int64, device=torch_dev), torch.tensor(ib, dtype=torch.int64, device=torch_dev),
+           torch.tensor(u, dtype=torch.float64, device=torch_dev))
+    E0 = st.energy.cpu().numpy().copy()
+    g.ctx.pt_run_chunk([st.job(accumulate=True, inj=inj)], steps, precision, 0)
+    torch.cuda.synchronize()
+    occ_ref = occ0.copy(); E_ref = np.array([o.energy(x) for x in occ0])
+    assert np.max(np.abs(E_ref - E0)) < 1e-9
+    traj_ref = np.zeros((steps // si, R)); s1 = np.zeros(R); s2 = np.zeros(R)
+    nacc_ref, emin_ref = o.run_chunk(occ_ref, E_ref, betas, steps, traj=traj_ref, sample_interval=si, accumulate=True,
+                                     e_ref=E0.copy(), s1=s1, s2=s2, e_min=E0.copy(), inj=(ia, ib, u))
+    occ_gpu = st.occ.cpu().numpy()
+    tol = 1e-9 if precision == 0 else 1e-4
+    mism = [r for r in range(R) if not np.array_equal(occ_gpu[r], occ_ref[r])]
+    assert not mism, f"occupations differ for replicas {mism}"
+    assert np.array_equal(st.n_accept.cpu().numpy(), nacc_ref)
+    assert np.max(np.abs(st.energy.cpu().numpy() - E_ref)) < tol
+    assert np.max(np.abs(st.traj.cpu().numpy() - traj_ref)) < tol
+    # running total == full recompute
+    E_full = np.array([o.energy(x) for x in occ_gpu])
+    assert np.max(np.abs(st.energy.cpu().numpy() - E_full)) / N < 1e-9 * (1 if precision == 0 else 1e3)
+
+
+def test_chunk_philox_stream_matches_oracle_replay(torch_dev, ptni_ce, ptni_template):
+    """the kernel's own Philox proposals replayed by the oracle give the same trajectory."""
+    import torch
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
+    g, o, sig = _setup(ptni_ce, ptni_template, 4)
+    R, steps = 16, 500
+    rng = np.random.default_rng(2)
+    occ0 = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
+    betas = (1.0 / (8.617333e-5 * np.geomspace(300, 1400, R))).astype(np.float32).astype(np.float64)
+    st = PTJobState(g, occ0, betas, seed=123456789, sample_interval=50, samples_per_chunk=steps // 50)
+    st.step.fill_(1000)
+    g.ctx.pt_run_chunk([st.job()], steps, 0, 0)
+    torch.cuda.synchronize()
+    occ_ref = occ0.copy(); E_ref = np.array([o.energy(x) for x in occ0])
+    nacc, _ = o.run_chunk(occ_ref, E_ref, betas, steps, seed=123456789, step0=1000)
+    assert np.array_equal(st.occ.cpu().numpy(), occ_ref)
+    assert np.array_equal(st.n_accept.cpu().numpy(), nacc)
+    assert np.max(np.abs(st.energy.cpu().numpy() - E_ref)) < 1e-9
+    assert np.all(st.step.cpu().numpy() == 1000 + steps)
+
+
+@pytest.mark.parametrize("R,parity", [(8, 0), (8, 1), (7, 1), (32, -1), (2, 0)])
+def test_exchange_matches_oracle(torch_dev, ptni_ce, ptni_template, R, parity):
+    import torch
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
+    from oracle.ce_oracle import exchange
+    g, o, sig = _setup(ptni_ce, ptni_template, 2)
+    rng = np.random.default_rng(R)
+    occ0 = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
+    betas = (1.0 / (8.617333e-5 * np.geomspace(300, 1400, R))).astype(np.float32).astype(np.float64)
+    st = PTJobState(g, occ0, betas, seed=99, sample_interval=1, samples_per_chunk=1)
+    E = st.energy.cpu().numpy()
+    # make exchanges likely: energies spread over a few kT
+    Eset = E.min() + rng.normal(0, 0.05, R)
+    st.energy.copy_(torch.tensor(Eset, device=torch_dev))
+    inj_u = None if parity < 0 else torch.tensor(rng.random(R), device=torch_dev)
+    ej = st.exchange_job(17, inj_parity=parity, inj_u=inj_u)
+    g.ctx.pt_exchange([ej])
+    torch.cuda.synchronize()
+    ros = np.arange(R, dtype=np.int32); brep = np.zeros(R)
+    n = exchange(Eset, ros, betas, brep, seed=st.seed ^ 0x9E3779B97F4A7C15, ex_index=17, inj_parity=parity,
+                 inj_u=None if inj_u is None else inj_u.cpu().numpy())
+    assert np.array_equal(st.rep_of_slot.cpu().numpy(), ros)
+    assert np.array_equal(st.beta_rep.cpu().numpy(), brep)
+    assert int(st.n_swaps.item()) == n
+    sor = np.empty(R, np.int32); sor[ros] = np.arange(R)
+    assert np.array_equal(st.slot_of_rep.cpu().numpy(), sor)
+    assert np.array_equal(st.traj[-1].cpu().numpy(), Eset[ros])
