@@ -92,7 +92,18 @@ class CETemplate:
                     if key not in slots:
                         slots[key] = len(slots)
                     ent_slot[b, j, m] = slots[key]
-            W.append(sorted(slots.items(), key=lambda kv: kv[1]))
+            # order slots by distance from the site (slot 0 = the site): compact clusters then use the first slots,
+            # which the swap kernel's bit-packed windows rely on (every triplet / quadruplet in the first 64 slots)
+            fb = prim.get_scaled_positions()
+            x0 = fb[b] @ prim.cell
+            dist = {key: float(np.linalg.norm((fb[key[0]] + np.asarray(key[1])) @ prim.cell - x0)) for key in slots}
+            order = sorted(slots, key=lambda key: (round(dist[key], 6), key[0], key[1]))
+            remap = {slots[key]: i for i, key in enumerate(order)}
+            for j in range(len(ent[b])):
+                for m in range(mo):
+                    if ent_slot[b, j, m] != SLOT_ONE:
+                        ent_slot[b, j, m] = remap[int(ent_slot[b, j, m])]
+            W.append([(key, i) for i, key in enumerate(order)])
         self.W_b = np.array([len(w) for w in W], np.int32)
         Wmax = int(self.W_b.max())
         if Wmax >= SLOT_ONE:

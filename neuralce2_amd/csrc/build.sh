@@ -3,7 +3,7 @@
 set -euo pipefail
 here="$(cd "$(dirname "$0")" && pwd)"
 out="${here}/../libneuralce_hip.so"
-/opt/rocm/bin/hipcc --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
+/opt/rocm/bin/hipcc --offload-arch=gfx950 -O3 -std=c++20 -fPIC -shared \
     -Xclang -target-feature -Xclang -packed-fp32-ops \
     -Wall -Wno-unused-function \
     -o "${out}.tmp" "${here}/neuralce_hip.hip"

@@ -13,8 +13,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -37,6 +39,19 @@ static int set_err(int code, const std::string &msg) {
     } while (0)
 
 extern "C" const char *nce_last_error(void) { return g_err.c_str(); }
+
+// Pointers read from job / model descriptors are generic; the hot ones are cast to the global address space so
+// the compiler emits global_load / global_store (flat accesses also count against lgkmcnt, so every LDS wait would
+// drain the in-flight window prefetches).
+#define NCE_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline NCE_GLOBAL T *gptr(T *p) {
+    return (NCE_GLOBAL T *)p;
+}
+template <typename T>
+__device__ inline const NCE_GLOBAL T *gptr(const T *p) {
+    return (const NCE_GLOBAL T *)p;
+}
 
 // ───────────────────────────────────────────── Philox ─────────────────────────────────────────────
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11).
@@ -77,6 +92,16 @@ struct ModelDev {
     const int4 *rows;           // [Rt*mo]
     const int32_t *row_bf;      // [Rt*mo]
     const double *row_coef;     // [Rt]
+    // division by d1*d2 and d2 of non-negative int32 values (Granlund-Montgomery: q = (hi + ((x - hi) >> 1)) >> sh)
+    uint32_t div12_m, div12_s, div2_m, div2_s;
+    // binary fast path (one basis site, point functions -1 / +1, window <= 192 slots); index = log2(waves)
+    int32_t fast_nit[4];                // iterations of 64*waves entries, 0 = tile not available
+    int32_t fast_nit0;                  // 64-entry schedule: iterations on window word 0 only (the rest <= 4)
+    const uint8_t *fast_word[4];        // [nit] word (1 or 2) of the second mask, 0 = word 0 only
+    const unsigned long long *fast_m0[4];  // [nit][64*waves] bits of members in window word 0
+    const unsigned long long *fast_mk[4];  // [nit][64*waves] bits in window word fast_word[it]
+    const double *fast_c64[4];          // [nit][64*waves] coefficients (0 for padding)
+    const float *fast_c32[4];
 };
 
 struct nce_ctx {
@@ -176,6 +201,12 @@ __device__ inline Real block_sum(Real v, Real *red) {
 }
 
 // ───────────────────────────────────────────── swap energy change ─────────────────────────────────
+__device__ inline uint32_t udiv_magic(uint32_t x, uint32_t m, uint32_t sh, uint32_t d) {
+    if (m == 0) return d == 1 ? x : x >> sh;                          // d == 1
+    const uint32_t hi = __umulhi(x, m);
+    return (hi + ((x - hi) >> 1)) >> sh;
+}
+
 struct Coord {
     int32_t b, k0, k1, k2;
 };
@@ -211,7 +242,7 @@ struct Real2 {
 //   contains ia contributes nothing (the reference skips those entries, ce_gpu_implicit.py:189-191).
 template <typename Real, int BS>
 __device__ inline void fill_values(const ModelDev &m, const int4 *s_win, Real2<Real> *valA, Real2<Real> *valB,
-                                   const int8_t *__restrict__ occ, int64_t ia, int64_t ib, int sa, int sb) {
+                                   const NCE_GLOBAL int8_t *__restrict__ occ, int64_t ia, int64_t ib, int sa, int sb) {
     const Coord ca = site_coord(m, ia), cb = site_coord(m, ib);
     const int Wa = m.W_b[ca.b], Wb = m.W_b[cb.b];
     for (int w = threadIdx.x; w < Wa + Wb; w += BS) {
@@ -313,6 +344,21 @@ __device__ inline Proposal draw_proposal(const ModelDev &m, uint64_t seed, uint6
     return p;
 }
 
+// Same stream as draw_proposal for one active sublattice with one basis site (site index = cell index): the
+// sublattice and basis draws of y2, y3 are trivial there, so only n_cells is needed (no table reads).
+__device__ inline Proposal draw_proposal_1b(uint64_t seed, uint64_t step, uint32_t rep, uint64_t n_cells) {
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint32_t x[4] = {(uint32_t)step, (uint32_t)(step >> 32), rep, 0u};
+    uint32_t y[4] = {(uint32_t)step, (uint32_t)(step >> 32), rep, 1u};
+    philox4x32_10(x, k0, k1);
+    philox4x32_10(y, k0, k1);
+    Proposal p;
+    p.ia = (int64_t)umulhi64(((uint64_t)x[1] << 32) | x[0], n_cells);
+    p.ib = (int64_t)umulhi64(((uint64_t)x[3] << 32) | x[2], n_cells);
+    p.u = ((double)((((uint64_t)y[1] << 32) | y[0]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    return p;
+}
+
 // ───────────────────────────────────────────── chunk kernel ───────────────────────────────────────
 template <typename Real, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void ce_chunk_kernel(const ModelDev *__restrict__ models,
@@ -334,7 +380,7 @@ __global__ __launch_bounds__(64 * WAVES) void ce_chunk_kernel(const ModelDev *__
     load_model_lds<Real, BS>(m, s_slot, s_coef, s_win, valA, valB);
     __syncthreads();
 
-    int8_t *occ = job.occ + (int64_t)r * job.n_sites;
+    NCE_GLOBAL int8_t *occ = gptr(job.occ) + (int64_t)r * job.n_sites;
     double E = job.energy[r];
     const double beta = job.beta[r];
     const float beta_f = (float)beta;
@@ -403,6 +449,383 @@ __global__ __launch_bounds__(64 * WAVES) void ce_chunk_kernel(const ModelDev *__
     }
 }
 
+// ───────────────────────────────────────────── binary fast chunk kernel ──────────────────────────
+// Binary point functions (-1 / +1): a cluster's product is (-1)^(number of members whose species is 0), i.e. the
+// parity of the AND of the window's spin bits with the cluster's member mask.  Away from each other and from their
+// own periodic images, swapping two different species flips exactly the centre of each window, so
+//   dE = -2 * factor * (h_a + h_b),   h_x = sum over the entries of x of coef * (-1)^parity(bits_x & mask).
+// Windows are gathered D steps ahead of use (Philox proposals are independent of the state); at use, each thread
+// patches its prefetched slots with the commits of the D steps in between.  Steps whose windows contain the other
+// site or an image of their own centre take the general value-table path.
+template <typename Real>
+__device__ inline Real signed_coef(Real c, uint32_t parity);
+template <>
+__device__ inline float signed_coef<float>(float c, uint32_t p) {
+    return __int_as_float(__float_as_int(c) ^ (int)(p << 31));
+}
+template <>
+__device__ inline double signed_coef<double>(double c, uint32_t p) {
+    return __longlong_as_double(__double_as_longlong(c) ^ ((long long)p << 63));
+}
+
+struct FastLds {
+    size_t slot, coef, valA, valB, red, bits, ctr, near, cm, total;
+};
+
+template <typename Real>
+__host__ __device__ inline FastLds fast_lds_layout(int J, int Wcap, int waves) {
+    FastLds L;
+    size_t o = 0;
+    L.slot = o; o = align16(o + (size_t)J * sizeof(uint2));
+    L.coef = o; o = align16(o + (size_t)J * sizeof(Real));
+    L.valA = o; o = align16(o + (size_t)Wcap * 2 * sizeof(Real));
+    L.valB = o; o = align16(o + (size_t)Wcap * 2 * sizeof(Real));
+    L.red = o;  o = align16(o + (size_t)waves * sizeof(double));
+    L.bits = o; o = align16(o + 2 * 6 * sizeof(unsigned long long));
+    L.ctr = o;  o = align16(o + 2 * 2 * sizeof(int));
+    L.near = o; o = align16(o + 2 * (size_t)waves * sizeof(int));
+    L.cm = o;   o = align16(o + 4 * 16 * sizeof(int4));               // commit ring, up to 4 * 16 steps
+    L.total = o;
+    return L;
+}
+
+template <int WAVES>
+struct WaveIdx {
+    static constexpr int value = WAVES == 1 ? 0 : WAVES == 2 ? 1 : WAVES == 4 ? 2 : 3;
+};
+
+// Speculative parallel steps: wave w of a W-wave workgroup evaluates step t0 + w of the same replica from the
+// current occupations; the window loads of all W steps are in flight together.  A uniform pass then applies the
+// steps in order (Metropolis, running total, statistics); a step whose windows contain a site changed by an
+// earlier accepted step of the same group is re-evaluated on patched species before its decision, so the result is
+// exactly the sequential chain.
+constexpr int SPEC_MAXIT = 36;       // 64-entry iterations with one window word (+ up to 4 with a second word)
+
+struct SpecRec {
+    double dE;
+    int sa, sb, valid, conflict;
+};
+
+template <typename Real>
+struct SpecLds {
+    size_t slot, coef, val, prop, u, rec, commit, total;
+};
+
+template <typename Real>
+__host__ __device__ inline SpecLds<Real> spec_lds_layout(int J, int Wcap, int waves) {
+    SpecLds<Real> L;
+    size_t o = 0;
+    L.slot = o;   o = align16(o + (size_t)J * sizeof(uint2));
+    L.coef = o;   o = align16(o + (size_t)J * sizeof(Real));
+    L.val = o;    o = align16(o + (size_t)waves * 2 * Wcap * 2 * sizeof(Real));
+    L.prop = o;   o = align16(o + (size_t)waves * sizeof(int2));
+    L.u = o;      o = align16(o + (size_t)waves * sizeof(double));
+    L.rec = o;    o = align16(o + (size_t)waves * sizeof(SpecRec));
+    L.commit = o; o = align16(o + (size_t)waves * sizeof(int4));
+    L.total = o;
+    return L;
+}
+
+template <typename Real, int W, bool INJ>
+__global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *__restrict__ models,
+                                                               const nce_pt_job *__restrict__ jobs,
+                                                               const int2 *__restrict__ items, int64_t n_steps) {
+    constexpr int BS = 64 * W;
+    constexpr int NS = 3;                              // window slots per lane: lane, lane + 64, lane + 128
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int2 it = items[blockIdx.x];
+    const nce_pt_job job = jobs[it.x];
+    const ModelDev &mg = models[job.model];
+    const int r = it.y;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int J = mg.J, Wcap = mg.Wcap, Wn = mg.W_b[0], Jb0 = mg.J_b[0];
+    const SpecLds<Real> L = spec_lds_layout<Real>(J, Wcap, W);
+    uint2 *s_slot = (uint2 *)(smem + L.slot);
+    Real *s_coef = (Real *)(smem + L.coef);
+    Real2<Real> *valA = (Real2<Real> *)(smem + L.val) + (size_t)w * 2 * Wcap;
+    Real2<Real> *valB = valA + Wcap;
+    int2 *s_prop = (int2 *)(smem + L.prop);
+    double *s_u = (double *)(smem + L.u);
+    SpecRec *s_rec = (SpecRec *)(smem + L.rec);
+    int4 *s_commit = (int4 *)(smem + L.commit);
+    for (int i = tid; i < J; i += BS) {
+        s_slot[i] = mg.ent_slot[i];
+        if constexpr (sizeof(Real) == 8) s_coef[i] = mg.ent_coef[i];
+        else s_coef[i] = mg.ent_coef_f[i];
+    }
+    if (lane == 0) {
+        valA[Wcap - 1].o = valA[Wcap - 1].n = (Real)1;
+        valB[Wcap - 1].o = valB[Wcap - 1].n = (Real)1;
+    }
+    // entry masks of this lane (64-entry schedule): nit0 iterations on window word 0, then ntail with a second word
+    const int nit0 = mg.fast_nit0, ntail = mg.fast_nit[0] - mg.fast_nit0;
+    unsigned long long m0[SPEC_MAXIT], m0t[4], mkt[4];
+    Real cf[SPEC_MAXIT], cft[4];
+    int kwt[4];
+    {
+        const NCE_GLOBAL unsigned long long *gm0 = gptr(mg.fast_m0[0]), *gmk = gptr(mg.fast_mk[0]);
+#pragma unroll
+        for (int i = 0; i < SPEC_MAXIT; ++i) {
+            m0[i] = 0; cf[i] = (Real)0;
+            if (i < nit0) {
+                m0[i] = gm0[i * 64 + lane];
+                if constexpr (sizeof(Real) == 8) cf[i] = gptr(mg.fast_c64[0])[i * 64 + lane];
+                else cf[i] = gptr(mg.fast_c32[0])[i * 64 + lane];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            m0t[i] = 0; mkt[i] = 0; cft[i] = (Real)0; kwt[i] = 0;
+            if (i < ntail) {
+                const int ii = nit0 + i;
+                m0t[i] = gm0[ii * 64 + lane];
+                mkt[i] = gmk[ii * 64 + lane];
+                if constexpr (sizeof(Real) == 8) cft[i] = gptr(mg.fast_c64[0])[ii * 64 + lane];
+                else cft[i] = gptr(mg.fast_c32[0])[ii * 64 + lane];
+                kwt[i] = gptr(mg.fast_word[0])[ii];
+            }
+        }
+    }
+    int4 wofs[NS];
+    bool wok[NS];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        const int slot = lane + 64 * j;
+        wok[j] = slot < Wn;
+        wofs[j] = make_int4(0, 0, 0, 0);
+        if (wok[j]) {
+            const NCE_GLOBAL int *wp = (const NCE_GLOBAL int *)gptr(mg.win) + 4 * slot;
+            wofs[j] = make_int4(wp[0], wp[1], wp[2], wp[3]);
+        }
+    }
+    const int64_t n_cells = mg.n_cells;
+    const int d0 = mg.d0, d1 = mg.d1, d2 = mg.d2;
+    const uint32_t m12 = mg.div12_m, s12 = mg.div12_s, m2 = mg.div2_m, sh2 = mg.div2_s;
+    const uint32_t d12 = (uint32_t)(d1 * d2);
+    const double factor = mg.factor;
+    __syncthreads();
+
+    NCE_GLOBAL int8_t *occ = gptr(job.occ) + (int64_t)r * job.n_sites;
+    const NCE_GLOBAL int64_t *inj_ia = gptr(job.inj_ia), *inj_ib = gptr(job.inj_ib);
+    const NCE_GLOBAL double *inj_u = gptr(job.inj_u);
+    NCE_GLOBAL double *traj = gptr(job.traj);
+    double E = job.energy[r];
+    const double beta = job.beta[r];
+    const float beta_f = (float)beta;
+    const int slot_r = job.slot_of_rep[r];
+    const int64_t step0_v = job.step[r];
+    const uint64_t step0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(step0_v >> 32)) << 32) |
+                           (uint32_t)__builtin_amdgcn_readfirstlane((int)step0_v);
+    const uint64_t seed = job.seed;
+    double emin = job.e_min[r];
+    const double eref = job.accumulate ? job.e_ref[r] : 0.0;
+    double s1 = 0.0, s2 = 0.0;
+    int64_t nacc = 0;
+    const int64_t inj_base = (int64_t)r * n_steps;
+
+    // evaluation of this wave's step from (patched) species in registers
+    auto evaluate = [&](int ia, int ib, const int (&sA)[NS], const int (&sB)[NS], const int (&pA)[NS],
+                        const int (&pB)[NS], int sa, int sb, bool near) __attribute__((always_inline)) -> double {
+        Real sum;
+        if (!near) {
+            unsigned long long WA[3], WB[3];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                WA[j] = __ballot(wok[j] && pA[j] == 0);
+                WB[j] = __ballot(wok[j] && pB[j] == 0);
+            }
+            Real acc = (Real)0;
+#pragma unroll
+            for (int i = 0; i < SPEC_MAXIT; ++i) {
+                if (i < nit0) {
+                    acc += signed_coef<Real>(cf[i], (uint32_t)__popcll(WA[0] & m0[i]) & 1u);
+                    acc += signed_coef<Real>(cf[i], (uint32_t)__popcll(WB[0] & m0[i]) & 1u);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (i < ntail) {
+                    const unsigned long long a1 = kwt[i] == 1 ? WA[1] : WA[2], b1 = kwt[i] == 1 ? WB[1] : WB[2];
+                    acc += signed_coef<Real>(cft[i], (uint32_t)__popcll((WA[0] & m0t[i]) ^ (a1 & mkt[i])) & 1u);
+                    acc += signed_coef<Real>(cft[i], (uint32_t)__popcll((WB[0] & m0t[i]) ^ (b1 & mkt[i])) & 1u);
+                }
+            }
+            sum = (Real)-2 * wave_sum(acc);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                if (wok[j]) {
+                    const int slot = lane + 64 * j;
+                    int na = pA[j], nb = pB[j];
+                    if (sA[j] == ia) na = sb;
+                    if (sA[j] == ib) na = sa;
+                    if (sB[j] == ia) nb = sb;
+                    if (sB[j] == ib) nb = sa;
+                    Real2<Real> va, vb;
+                    va.o = pA[j] ? (Real)1 : (Real)-1;
+                    va.n = na ? (Real)1 : (Real)-1;
+                    vb.o = pB[j] ? (Real)1 : (Real)-1;
+                    vb.n = nb ? (Real)1 : (Real)-1;
+                    if (sB[j] == ia) { vb.o = (Real)0; vb.n = (Real)0; }
+                    valA[slot] = va;
+                    valB[slot] = vb;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            Real acc = (Real)0;
+            for (int j = lane; j < Jb0; j += 64) {
+                const uint2 e = s_slot[j];
+                acc += s_coef[j] * (entry_delta(valA, e) + entry_delta(valB, e));
+            }
+            sum = wave_sum(acc);
+        }
+        if constexpr (sizeof(Real) == 8) return (double)sum * factor;
+        else return (double)((float)sum * (float)factor);
+    };
+
+    for (int64_t t0 = 0; t0 < n_steps; t0 += W) {
+        const int64_t t = t0 + w;
+        const bool active = t < n_steps;
+        // 1. proposal of this wave's step and its window loads
+        int ia, ib;
+        double u;
+        if constexpr (INJ) {
+            const int64_t tc = active ? t : n_steps - 1;
+            ia = (int)inj_ia[inj_base + tc]; ib = (int)inj_ib[inj_base + tc]; u = inj_u[inj_base + tc];
+        } else {
+            const Proposal p = draw_proposal_1b(seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)n_cells);
+            ia = (int)p.ia; ib = (int)p.ib; u = p.u;
+        }
+        ia = __builtin_amdgcn_readfirstlane(ia);
+        ib = __builtin_amdgcn_readfirstlane(ib);
+        const int a0 = (int)udiv_magic((uint32_t)ia, m12, s12, d12), arem = ia - a0 * (int)d12;
+        const int a1 = (int)udiv_magic((uint32_t)arem, m2, sh2, (uint32_t)d2), a2 = arem - a1 * d2;
+        const int b0 = (int)udiv_magic((uint32_t)ib, m12, s12, d12), brem = ib - b0 * (int)d12;
+        const int b1 = (int)udiv_magic((uint32_t)brem, m2, sh2, (uint32_t)d2), b2 = brem - b1 * d2;
+        int sA[NS], sB[NS], pA[NS], pB[NS];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            int k0 = a0 + wofs[j].y, k1 = a1 + wofs[j].z, k2 = a2 + wofs[j].w;
+            k0 -= (k0 >= d0) ? d0 : 0; k1 -= (k1 >= d1) ? d1 : 0; k2 -= (k2 >= d2) ? d2 : 0;
+            sA[j] = wok[j] ? (k0 * d1 + k1) * d2 + k2 : -1;
+            k0 = b0 + wofs[j].y; k1 = b1 + wofs[j].z; k2 = b2 + wofs[j].w;
+            k0 -= (k0 >= d0) ? d0 : 0; k1 -= (k1 >= d1) ? d1 : 0; k2 -= (k2 >= d2) ? d2 : 0;
+            sB[j] = wok[j] ? (k0 * d1 + k1) * d2 + k2 : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            pA[j] = wok[j] ? (int)occ[sA[j]] : 1;
+            pB[j] = wok[j] ? (int)occ[sB[j]] : 1;
+        }
+        if (lane == 0) { s_prop[w] = make_int2(ia, ib); s_u[w] = u; }
+        __syncthreads();
+        // 2. conflicts with the earlier steps of the group, centres, near flag, energy change
+        int conflict = 0;
+        for (int q = 0; q < w; ++q) {
+            const int2 pq = s_prop[q];
+            bool hit = false;
+#pragma unroll
+            for (int j = 0; j < NS; ++j)
+                hit |= (sA[j] == pq.x) | (sA[j] == pq.y) | (sB[j] == pq.x) | (sB[j] == pq.y);
+            if (__ballot(hit) != 0ull) conflict |= 1 << q;
+        }
+        bool nr = false;
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
+            if (wok[j] && (lane + 64 * j) > 0) nr |= (sA[j] == ia) | (sA[j] == ib) | (sB[j] == ib) | (sB[j] == ia);
+        const bool near = __ballot(nr) != 0ull;
+        const int sa = __builtin_amdgcn_readfirstlane(pA[0]), sb = __builtin_amdgcn_readfirstlane(pB[0]);
+        const bool valid = active && sa != sb;
+        double dE = 0.0;
+        if (valid) dE = evaluate(ia, ib, sA, sB, pA, pB, sa, sb, near);
+        if (lane == 0) {
+            SpecRec rec;
+            rec.dE = dE; rec.sa = sa; rec.sb = sb; rec.valid = valid; rec.conflict = conflict;
+            s_rec[w] = rec;
+        }
+        __syncthreads();
+        // 3. decisions in step order (every wave, same values)
+        int acc_mask = 0, n_commit = 0;
+        for (int i = 0; i < W; ++i) {
+            const int64_t ti = t0 + i;
+            if (ti >= n_steps) break;
+            SpecRec rec = s_rec[i];
+            if (rec.conflict & acc_mask) {
+                // re-evaluate step i on the species after the accepted steps before it (wave i); the barrier makes
+                // the commit list written by thread 0 visible
+                __syncthreads();
+                if (w == i) {
+                    int qA[NS], qB[NS];
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) { qA[j] = pA[j]; qB[j] = pB[j]; }
+                    for (int c = 0; c < n_commit; ++c) {
+                        const int4 cm = s_commit[c];
+#pragma unroll
+                        for (int j = 0; j < NS; ++j) {
+                            if (sA[j] == cm.x) qA[j] = cm.z;
+                            if (sA[j] == cm.y) qA[j] = cm.w;
+                            if (sB[j] == cm.x) qB[j] = cm.z;
+                            if (sB[j] == cm.y) qB[j] = cm.w;
+                        }
+                    }
+                    const int sa2 = __builtin_amdgcn_readfirstlane(qA[0]), sb2 = __builtin_amdgcn_readfirstlane(qB[0]);
+                    const bool valid2 = sa2 != sb2;
+                    double dE2 = 0.0;
+                    if (valid2) dE2 = evaluate(ia, ib, sA, sB, qA, qB, sa2, sb2, near);
+                    if (lane == 0) {
+                        SpecRec r2;
+                        r2.dE = dE2; r2.sa = sa2; r2.sb = sb2; r2.valid = valid2; r2.conflict = rec.conflict;
+                        s_rec[i] = r2;
+                    }
+                }
+                __syncthreads();
+                rec = s_rec[i];
+            }
+            if (rec.valid) {
+                const double ui = s_u[i];
+                bool accept;
+                if constexpr (sizeof(Real) == 8) accept = log(ui) < -beta * rec.dE;
+                else accept = logf((float)ui) < -(beta_f * (float)rec.dE);
+                if (accept) {
+                    const int2 pi = s_prop[i];
+                    E += rec.dE;
+                    ++nacc;
+                    acc_mask |= 1 << i;
+                    if (tid == 0) {
+                        s_commit[n_commit] = make_int4(pi.x, pi.y, rec.sb, rec.sa);
+                        occ[pi.x] = (int8_t)rec.sb;
+                        occ[pi.y] = (int8_t)rec.sa;
+                    }
+                    ++n_commit;
+                    // the commit list is read by a later re-evaluation only after that step's barrier
+                }
+            }
+            emin = fmin(emin, E);
+            if (job.accumulate) {
+                const double x = E - eref;
+                s1 += x;
+                s2 += x * x;
+            }
+            if (traj && ((ti + 1) % job.sample_interval) == 0 && tid == 0)
+                traj[((ti + 1) / job.sample_interval - 1) * job.n_replicas + slot_r] = E;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        job.energy[r] = E;
+        job.e_min[r] = emin;
+        job.n_accept[r] += nacc;
+        job.step[r] = (int64_t)(step0 + (uint64_t)n_steps);
+        if (job.accumulate) {
+            job.s1[r] += s1;
+            job.s2[r] += s2;
+        }
+    }
+}
+
 // ───────────────────────────────────────────── single delta kernel ────────────────────────────────
 template <typename Real, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void ce_delta_kernel(const ModelDev *__restrict__ models, int model,
@@ -422,7 +845,7 @@ __global__ __launch_bounds__(64 * WAVES) void ce_delta_kernel(const ModelDev *__
     load_model_lds<Real, BS>(m, s_slot, s_coef, s_win, valA, valB);
     __syncthreads();
     const int r = blockIdx.x;
-    const int8_t *occ = occ_all + (int64_t)r * N;
+    const NCE_GLOBAL int8_t *occ = gptr(occ_all) + (int64_t)r * N;
     const int64_t ia = ia_all[r], ib = ib_all[r];
     const int sa = occ[ia], sb = occ[ib];
     fill_values<Real, BS>(m, s_win, valA, valB, occ, ia, ib, sa, sb);
@@ -609,6 +1032,84 @@ static int sync_models(nce_ctx *c) {
     return NCE_OK;
 }
 
+static const int FAST_MAXIT[4] = {40, 24, 12, 8};
+
+// magic numbers for unsigned division by d (1 <= d < 2^31) valid for every 32-bit dividend
+static void magic_u32(uint32_t d, uint32_t *m, uint32_t *sh) {
+    uint32_t l = 0;
+    while (((uint64_t)1 << l) < d) ++l;                              // l = ceil(log2 d)
+    const uint64_t mm = ((((uint64_t)1 << l) - d) << 32) / d + 1;
+    *m = (uint32_t)mm;
+    *sh = l == 0 ? 0 : l - 1;
+    if (l == 0) { *m = 0; *sh = 0; }
+}
+
+// Fast-path tables: entries scheduled in iterations of 64*waves, one second window word per iteration.
+static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m) {
+    for (int w = 0; w < 4; ++w) m.fast_nit[w] = 0;
+    if (d->n_basis != 1 || d->n_active != 1 || d->active_nbasis[0] != 1 || d->max_window > 192) return NCE_OK;
+    for (int sl = 0; sl < d->n_sublattices; ++sl) {
+        const double *pf = d->pf + sl * 4;
+        if (pf[0] != 1.0 || pf[1] != 1.0 || pf[2] != -1.0 || pf[3] != 1.0) return NCE_OK;
+    }
+    const int Jb = d->entries_per_basis[0];
+    std::vector<int> grp[3];
+    std::vector<unsigned long long> A(Jb), B(Jb);
+    for (int j = 0; j < Jb; ++j) {
+        unsigned long long a = 0, b = 0;
+        int k = 0, centre = 0;
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t sl = d->entry_slot[(size_t)j * 4 + q];
+            if (sl == 0xFFFFu) continue;
+            if (sl == 0) ++centre;
+            const int word = (int)(sl >> 6);
+            const unsigned long long bit = 1ull << (sl & 63);
+            if (word == 0) a |= bit;
+            else {
+                if (k && k != word) return NCE_OK;          // members in two upper words: general path only
+                k = word;
+                b |= bit;
+            }
+        }
+        if (centre != 1) return NCE_OK;
+        A[j] = a; B[j] = b;
+        grp[k].push_back(j);
+    }
+    for (int w = 0; w < 4; ++w) {
+        const int BS = 64 << w;
+        int nit = 0;
+        for (int k = 0; k < 3; ++k) nit += (int)((grp[k].size() + BS - 1) / BS);
+        if (nit < 1 || nit > FAST_MAXIT[w]) continue;
+        std::vector<uint8_t> word(nit);
+        std::vector<unsigned long long> m0((size_t)nit * BS, 0ull), mk((size_t)nit * BS, 0ull);
+        std::vector<double> c64((size_t)nit * BS, 0.0);
+        std::vector<float> c32((size_t)nit * BS, 0.0f);
+        int itn = 0;
+        for (int k = 0; k < 3; ++k) {
+            for (size_t base = 0; base < grp[k].size(); base += BS, ++itn) {
+                word[itn] = (uint8_t)k;
+                for (int t = 0; t < BS && base + t < grp[k].size(); ++t) {
+                    const int j = grp[k][base + t];
+                    const size_t idx = (size_t)itn * BS + t;
+                    m0[idx] = A[j]; mk[idx] = B[j];
+                    c64[idx] = d->entry_coef[j]; c32[idx] = (float)d->entry_coef[j];
+                }
+            }
+        }
+        if (w == 0) {
+            m.fast_nit0 = (int)((grp[0].size() + BS - 1) / BS);
+            if (nit - m.fast_nit0 > 4 || m.fast_nit0 > SPEC_MAXIT) continue;
+        }
+        UP(word.data(), word.size(), &m.fast_word[w]);
+        UP(m0.data(), m0.size(), &m.fast_m0[w]);
+        UP(mk.data(), mk.size(), &m.fast_mk[w]);
+        UP(c64.data(), c64.size(), &m.fast_c64[w]);
+        UP(c32.data(), c32.size(), &m.fast_c32[w]);
+        m.fast_nit[w] = nit;
+    }
+    return NCE_OK;
+}
+
 extern "C" int nce_ce_model_create(nce_ctx *c, const nce_ce_model_desc *d, int32_t *model_id) {
     if (!c || !d || !model_id) return set_err(NCE_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(c->device));
@@ -684,6 +1185,10 @@ extern "C" int nce_ce_model_create(nce_ctx *c, const nce_ce_model_desc *d, int32
     UP((const int4 *)d->rows, nR, &m.rows);
     UP(d->row_bf, nR, &m.row_bf);
     UP(d->row_coef, (size_t)m.Rt, &m.row_coef);
+    magic_u32((uint32_t)(m.d1 * m.d2), &m.div12_m, &m.div12_s);
+    magic_u32((uint32_t)m.d2, &m.div2_m, &m.div2_s);
+    if ((int64_t)m.d1 * m.d2 >= ((int64_t)1 << 31)) return set_err(NCE_ERR_UNSUPPORTED, "cell too large (d1*d2)");
+    if (int rcf = build_fast_tables(c, d, m)) return rcf;
     c->models.push_back(m);
     c->lds_chunk64.push_back(chunk_lds_layout<double>(m.nB, m.J, m.Wmax, m.Wcap, 8).total);
     c->lds_chunk32.push_back(chunk_lds_layout<float>(m.nB, m.J, m.Wmax, m.Wcap, 8).total);
@@ -798,6 +1303,18 @@ static int launch_chunk(nce_ctx *c, const nce_pt_job *d_jobs, const int2 *d_item
     return NCE_OK;
 }
 
+template <typename Real, int W>
+static int launch_chunk_spec(nce_ctx *c, const nce_pt_job *d_jobs, const int2 *d_items, int n_items,
+                             int64_t n_steps, size_t lds, hipStream_t s, bool inj) {
+    auto k = inj ? ce_chunk_spec_kernel<Real, W, true> : ce_chunk_spec_kernel<Real, W, false>;
+    if (int rc = allow_lds(k, lds)) return rc;
+    hipLaunchKernelGGL(k, dim3(n_items), dim3(64 * W), lds, s, c->d_models, d_jobs, d_items, n_steps);
+    HIP_TRY(hipGetLastError());
+    return NCE_OK;
+}
+
+static int wave_index(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
+
 extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps,
                                 int32_t precision, int32_t tile, void *stream) {
     if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
@@ -825,12 +1342,12 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
         lds_need = std::max(lds_need, precision == NCE_PREC_FP64 ? c->lds_chunk64[jb.model] : c->lds_chunk32[jb.model]);
     }
     if (lds_need > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "template does not fit in LDS");
-    int W = tile;
-    if (W == 0) {                                      // default tiling: fill the CUs with waves per replica
-        const int per_cu = (int)((items.size() + c->n_cu - 1) / c->n_cu);
-        W = per_cu >= 4 ? 1 : per_cu >= 2 ? 2 : 4;
-    }
-    if (W != 1 && W != 2 && W != 4 && W != 8) return set_err(NCE_ERR_ARG, "tile must be 0, 1, 2, 4 or 8");
+    if (tile != 0 && tile != 1 && tile != 2 && tile != 4 && tile != 8 && tile != 16)
+        return set_err(NCE_ERR_ARG, "tile must be 0, 1, 2, 4, 8 or 16");
+    const int per_cu = (int)((items.size() + c->n_cu - 1) / c->n_cu);
+    // general kernel: waves cooperating on one step of a replica; speculative kernel: steps evaluated in parallel
+    int W = tile == 0 ? (per_cu >= 4 ? 1 : per_cu >= 2 ? 2 : 4) : (tile > 8 ? 8 : tile);
+    const int tile_spec = (tile == 4 || tile == 8) ? tile : tile == 16 ? 8 : (per_cu >= 4 ? 4 : 8);
     // job table and work items in one staged buffer
     const size_t jb_bytes = align16(sizeof(nce_pt_job) * n_jobs);
     std::vector<unsigned char> buf(jb_bytes + items.size() * sizeof(int2));
@@ -849,6 +1366,31 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
         const ModelDev &m = c->models[jobs[j].model];
         lds = std::max(lds, precision == NCE_PREC_FP64 ? chunk_lds_layout<double>(m.nB, m.J, m.Wmax, m.Wcap, W).total
                                                        : chunk_lds_layout<float>(m.nB, m.J, m.Wmax, m.Wcap, W).total);
+    }
+    bool fast = true;
+    const bool inj = jobs[0].inj_ia != nullptr;
+    size_t lds_fast = 0;
+    int SW = tile_spec;
+    for (int j = 0; j < n_jobs; ++j) {
+        const ModelDev &m = c->models[jobs[j].model];
+        if (m.fast_nit[0] == 0 || m.n_sites >= ((int64_t)1 << 31)) fast = false;
+        if ((jobs[j].inj_ia != nullptr) != inj) fast = false;          // one stream kind per launch
+        lds_fast = std::max(lds_fast, precision == NCE_PREC_FP64 ? spec_lds_layout<double>(m.J, m.Wcap, SW).total
+                                                                 : spec_lds_layout<float>(m.J, m.Wcap, SW).total);
+    }
+    const char *fg = std::getenv("NCE_FORCE_GENERAL");               // tests: run the general kernel on any model
+    if (fg && fg[0] == '1') fast = false;
+    if (fast && lds_fast <= 160 * 1024) {
+        if (precision == NCE_PREC_FP64) {
+            switch (SW) {
+                case 4: return launch_chunk_spec<double, 4>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
+                default: return launch_chunk_spec<double, 8>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
+            }
+        }
+        switch (SW) {
+            case 4: return launch_chunk_spec<float, 4>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
+            default: return launch_chunk_spec<float, 8>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
+        }
     }
     if (precision == NCE_PREC_FP64) {
         switch (W) {

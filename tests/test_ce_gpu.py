@@ -81,8 +81,13 @@ def _near_tie(o, occ_before_path, *a):
     return False
 
 
-@pytest.mark.parametrize("n,R,steps,precision", [(4, 8, 400, 0), (8, 32, 300, 0), (2, 4, 500, 0), (4, 8, 400, 1)])
-def test_chunk_injected_streams_bitexact(torch_dev, ptni_ce, ptni_template, n, R, steps, precision):
+CHUNK_CASES = [(4, 8, 400, 0, 0), (8, 32, 300, 0, 0), (2, 4, 500, 0, 0), (4, 8, 400, 1, 0),
+               (8, 8, 301, 0, 4), (8, 8, 302, 0, 8), (8, 8, 303, 1, 4), (8, 8, 305, 1, 8),
+               (16, 6, 400, 0, 4), (16, 6, 400, 1, 8), (1, 4, 203, 0, 8), (2, 4, 203, 1, 4)]
+
+
+@pytest.mark.parametrize("n,R,steps,precision,tile", CHUNK_CASES)
+def test_chunk_injected_streams_bitexact(torch_dev, ptni_ce, ptni_template, n, R, steps, precision, tile):
     """same injected proposals and uniforms -> identical occupations; energies within tolerance."""
     import torch
     from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
@@ -92,13 +97,13 @@ def test_chunk_injected_streams_bitexact(torch_dev, ptni_ce, ptni_template, n, R
     occ0 = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
     betas = 1.0 / (8.617333e-5 * np.geomspace(300, 1400, R).astype(np.float32))
     betas = betas.astype(np.float32).astype(np.float64)
-    si = 10
+    si = 10 if steps % 10 == 0 else 1
     st = PTJobState(g, occ0, betas, seed=5, sample_interval=si, samples_per_chunk=steps // si)
     ia = rng.integers(0, N, (R, steps)); ib = rng.integers(0, N, (R, steps)); u = rng.random((R, steps))
     inj = (torch.tensor(ia, dtype=torch.int64, device=torch_dev), torch.tensor(ib, dtype=torch.int64, device=torch_dev),
            torch.tensor(u, dtype=torch.float64, device=torch_dev))
     E0 = st.energy.cpu().numpy().copy()
-    g.ctx.pt_run_chunk([st.job(accumulate=True, inj=inj)], steps, precision, 0)
+    g.ctx.pt_run_chunk([st.job(accumulate=True, inj=inj)], steps, precision, tile)
     torch.cuda.synchronize()
     occ_ref = occ0.copy(); E_ref = np.array([o.energy(x) for x in occ0])
     assert np.max(np.abs(E_ref - E0)) < 1e-9
@@ -112,23 +117,27 @@ def test_chunk_injected_streams_bitexact(torch_dev, ptni_ce, ptni_template, n, R
     assert np.array_equal(st.n_accept.cpu().numpy(), nacc_ref)
     assert np.max(np.abs(st.energy.cpu().numpy() - E_ref)) < tol
     assert np.max(np.abs(st.traj.cpu().numpy() - traj_ref)) < tol
-    # running total == full recompute
-    E_full = np.array([o.energy(x) for x in occ_gpu])
-    assert np.max(np.abs(st.energy.cpu().numpy() - E_full)) / N < 1e-9 * (1 if precision == 0 else 1e3)
+    # running total == full recompute (cells of one conventional cube hold clusters that contain a site twice
+    # through a periodic image; the reference's swap delta counts those once per member position, so the identity
+    # only holds from n = 2 on, for the reference and for this kernel alike)
+    if n >= 2:
+        E_full = np.array([o.energy(x) for x in occ_gpu])
+        assert np.max(np.abs(st.energy.cpu().numpy() - E_full)) / N < 1e-9 * (1 if precision == 0 else 1e3)
 
 
-def test_chunk_philox_stream_matches_oracle_replay(torch_dev, ptni_ce, ptni_template):
+@pytest.mark.parametrize("n,tile", [(4, 0), (8, 8), (8, 4), (16, 4), (16, 8)])
+def test_chunk_philox_stream_matches_oracle_replay(torch_dev, ptni_ce, ptni_template, n, tile):
     """the kernel's own Philox proposals replayed by the oracle give the same trajectory."""
     import torch
     from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
-    g, o, sig = _setup(ptni_ce, ptni_template, 4)
+    g, o, sig = _setup(ptni_ce, ptni_template, n)
     R, steps = 16, 500
     rng = np.random.default_rng(2)
     occ0 = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
     betas = (1.0 / (8.617333e-5 * np.geomspace(300, 1400, R))).astype(np.float32).astype(np.float64)
     st = PTJobState(g, occ0, betas, seed=123456789, sample_interval=50, samples_per_chunk=steps // 50)
     st.step.fill_(1000)
-    g.ctx.pt_run_chunk([st.job()], steps, 0, 0)
+    g.ctx.pt_run_chunk([st.job()], steps, 0, tile)
     torch.cuda.synchronize()
     occ_ref = occ0.copy(); E_ref = np.array([o.energy(x) for x in occ0])
     nacc, _ = o.run_chunk(occ_ref, E_ref, betas, steps, seed=123456789, step0=1000)
@@ -165,3 +174,56 @@ def test_exchange_matches_oracle(torch_dev, ptni_ce, ptni_template, R, parity):
     sor = np.empty(R, np.int32); sor[ros] = np.arange(R)
     assert np.array_equal(st.slot_of_rep.cpu().numpy(), sor)
     assert np.array_equal(st.traj[-1].cpu().numpy(), Eset[ros])
+
+
+def test_chunk_large_cell_fast_path(torch_dev):
+    """1,048,576-site cell at x = 0.25 (the benchmark workload, synthetic ECIs): a short Philox chunk of every
+    replica replayed by the oracle; running totals equal full recomputes."""
+    import torch
+    from neuralce2_amd.synthetic import synthetic_ce, fcc_cell
+    from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
+    from oracle.ce_oracle import OracleCE
+    ce = synthetic_ce(0)
+    at = fcc_cell(64, 0.25, 0)
+    g = GPUClusterExpansion(ce, at, energy_scale=float(len(at)))
+    o = OracleCE(g.info)
+    sig = g.encode(at).astype(np.int8)
+    R, steps = 4, 2000
+    betas = (1.0 / (8.617333e-5 * np.geomspace(300, 1400, R))).astype(np.float32).astype(np.float64)
+    for precision in (0, 1):
+        st = PTJobState(g, sig, betas, seed=3 + precision, sample_interval=100, samples_per_chunk=steps // 100)
+        g.ctx.pt_run_chunk([st.job()], steps, precision, 8)
+        torch.cuda.synchronize()
+        occ = np.ascontiguousarray(np.broadcast_to(sig, (R, len(sig))).copy())
+        E0 = o.energy(sig); E = np.full(R, E0)
+        o.run_chunk(occ, E, betas, steps, seed=3 + precision)
+        got = st.occ.cpu().numpy()
+        assert np.array_equal(got, occ)
+        tol = 1e-9 if precision == 0 else 1e-6 * len(sig) * 1e-3
+        assert np.max(np.abs(st.energy.cpu().numpy() - E)) < max(tol, 1e-9)
+        Efull = g.batched_energy(got)
+        assert np.max(np.abs(Efull - st.energy.cpu().numpy())) / len(sig) < 1e-9
+
+
+@pytest.mark.parametrize("n,tile,precision", [(4, 1, 0), (8, 2, 0), (8, 4, 1), (16, 8, 0), (2, 1, 1)])
+def test_general_kernel_injected_streams(torch_dev, ptni_ce, ptni_template, n, tile, precision, monkeypatch):
+    """the general (any-basis, value-table) chunk kernel on the same parity bar."""
+    import torch
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
+    monkeypatch.setenv("NCE_FORCE_GENERAL", "1")
+    g, o, sig = _setup(ptni_ce, ptni_template, n)
+    N = len(sig); R, steps = 6, 300
+    rng = np.random.default_rng(21)
+    occ0 = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
+    betas = (1.0 / (8.617333e-5 * np.geomspace(300, 1400, R))).astype(np.float32).astype(np.float64)
+    st = PTJobState(g, occ0, betas, seed=5, sample_interval=10, samples_per_chunk=steps // 10)
+    ia = rng.integers(0, N, (R, steps)); ib = rng.integers(0, N, (R, steps)); u = rng.random((R, steps))
+    inj = tuple(torch.tensor(x, device=torch_dev) for x in (ia, ib, u))
+    g.ctx.pt_run_chunk([st.job(inj=inj)], steps, precision, tile)
+    torch.cuda.synchronize()
+    occ_ref = occ0.copy(); E_ref = np.array([o.energy(x) for x in occ0])
+    nacc, _ = o.run_chunk(occ_ref, E_ref, betas, steps, inj=(ia, ib, u))
+    assert np.array_equal(st.occ.cpu().numpy(), occ_ref)
+    assert np.array_equal(st.n_accept.cpu().numpy(), nacc)
+    assert np.max(np.abs(st.energy.cpu().numpy() - E_ref)) < (1e-9 if precision == 0 else 1e-4)
