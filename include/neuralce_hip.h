@@ -137,6 +137,74 @@ int nce_pt_run_chunk(nce_ctx *ctx, const nce_pt_job *jobs, int32_t n_jobs, int64
  * reordering of the last sample after it (:808-810).  jobs: HOST array. */
 int nce_pt_exchange(nce_ctx *ctx, const nce_pt_exchange_job *jobs, int32_t n_jobs, void *stream);
 
+/* ── Neural cluster expansion (receptive-field local update) ─────────────────────────────────────────────────
+ * Model on one cell shape: layer-0 template (window of the sites sharing a cluster with a site, reverse entries with
+ * their channel and coefficient), shell partner offsets, and the network.  One basis site, binary species.
+ * Replaces setup_implicit / build_geometry / make_local_fns_v3 (neuralce/nce/mc_model.py:175-228,
+ * neuralce/nce/gpu_layer0_implicit.py:17-118, neuralce/nce/local_update_implicit.py:19-197). Host arrays. */
+typedef struct {
+    int32_t d[3];
+    int64_t n_cells;
+    int32_t n_window;                  /* W0: layer-0 window slots (slot 0 = the site) */
+    const int32_t *window;             /* [W0][4] (basis, k0, k1, k2) reduced mod d */
+    int32_t n_entries;                 /* J0 */
+    const uint16_t *entry_slot;        /* [J0][4] window slots of the members, 0xFFFF = padding */
+    const int32_t *entry_col;          /* [J0] layer-0 channel (1 + orbit) */
+    const double *entry_coef;          /* [J0] nA / (n_instances * n_cells * order) */
+    int32_t n_shells;                  /* P */
+    const int32_t *shell_count;        /* [P] partners per shell */
+    const int32_t *partners;           /* [sum P][4] (basis, k0, k1, k2) reduced mod d, grouped by shell */
+    int32_t c0;                        /* layer-0 channels (zerolet included) */
+    const double *mu, *sd, *lin;       /* [c0] standardisation and layer-0 readout */
+    int32_t n_layers;                  /* L */
+    const int32_t *layer_cin, *layer_d;/* [L] */
+    const double *w_self;              /* per layer [cin][d], concatenated */
+    const double *w_shell;             /* per layer [P][cin][d] */
+    const double *w_gate;              /* per layer [P][cin][d] or NULL */
+    const double *b, *gb;              /* per layer [d] (gb NULL without gate) */
+    int32_t activation;                /* 0: tanh (times the sigmoid gate if w_gate), 1: SiLU */
+    const double *head_w;              /* [d_L] */
+    double head_b;
+    int32_t readout;                   /* 0: mean over sites (+ head_b), 1: sum over sites (+ head_b) */
+    double de_scale;                   /* model energy change -> total eV (ys * N for the reference models) */
+    int32_t ball_cap[8];               /* capacity (sites) of ball_l for the union of two flips, l = 0..L */
+} nce_nn_model_desc;
+
+/* One NeuralCE job for nce_nn_run_chunk (device pointers). */
+typedef struct {
+    int32_t model;
+    int32_t n_replicas;
+    int64_t n_sites;
+    int8_t *occ;                /* [R][N] */
+    void *cache;                /* [R][nce_nn_cache_floats] FP32 (mixed) or FP64 caches: h0s, hidden layers, e */
+    void *scratch;              /* [R][nce_nn_scratch_floats] same type */
+    double *energy;             /* [R] running totals (model units, standardised) */
+    const double *beta;         /* [R] 1/(kB T) */
+    int64_t *step;              /* [R] Philox counter */
+    uint64_t seed;
+    int64_t *n_accept;          /* [R] */
+    double *traj;               /* [n_steps/sample_interval][R] or NULL */
+    int32_t sample_interval;
+    int32_t pad;
+    const int64_t *inj_ia, *inj_ib;   /* [R][n_steps] injected proposals or NULL */
+    const double *inj_u;
+} nce_nn_job;
+
+/* Per-replica sizes (elements of the precision's type) of the cache and the scratch of a NeuralCE model. */
+int nce_nn_model_create(nce_ctx *ctx, const nce_nn_model_desc *desc, int32_t *model_id);
+int nce_nn_sizes(nce_ctx *ctx, int32_t model, int64_t *cache_elems, int64_t *scratch_elems);
+
+/* Full caches and energies of R configurations (h0_full + every layer + energy_of; neuralce/nce/
+ * local_update_implicit.py:89-112); energy[r] = sum or mean of the site energies + head_b (model units). */
+int nce_nn_full_cache(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_replicas, int64_t n_sites,
+                      void *cache, double *energy, int32_t precision, void *stream);
+
+/* n_steps swap steps of every replica (local_delta + Metropolis + commit: neuralce/nce/mcmc.py:130-199,
+ * local_update_implicit.py:117-194); a same-species or same-site proposal is an accepted step with dE = 0 as in
+ * the reference (mcmc.py:147,179).  jobs: HOST array. */
+int nce_nn_run_chunk(nce_ctx *ctx, const nce_nn_job *jobs, int32_t n_jobs, int64_t n_steps, int32_t precision,
+                     void *stream);
+
 /* Philox4x32-10 of the library (known-answer checks). */
 void nce_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

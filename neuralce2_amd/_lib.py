@@ -43,6 +43,23 @@ class ExchangeJob(C.Structure):
                 ("ex_index", C.c_uint64), ("n_swaps", _p), ("inj_u", _p), ("traj_last", _p)]
 
 
+class NNModelDesc(C.Structure):
+    _fields_ = [("d", C.c_int32 * 3), ("n_cells", C.c_int64), ("n_window", C.c_int32), ("window", _p),
+                ("n_entries", C.c_int32), ("entry_slot", _p), ("entry_col", _p), ("entry_coef", _p),
+                ("n_shells", C.c_int32), ("shell_count", _p), ("partners", _p), ("c0", C.c_int32), ("mu", _p),
+                ("sd", _p), ("lin", _p), ("n_layers", C.c_int32), ("layer_cin", _p), ("layer_d", _p),
+                ("w_self", _p), ("w_shell", _p), ("w_gate", _p), ("b", _p), ("gb", _p), ("activation", C.c_int32),
+                ("head_w", _p), ("head_b", C.c_double), ("readout", C.c_int32), ("de_scale", C.c_double),
+                ("ball_cap", C.c_int32 * 8)]
+
+
+class NNJob(C.Structure):
+    _fields_ = [("model", C.c_int32), ("n_replicas", C.c_int32), ("n_sites", C.c_int64), ("occ", _p), ("cache", _p),
+                ("scratch", _p), ("energy", _p), ("beta", _p), ("step", _p), ("seed", C.c_uint64),
+                ("n_accept", _p), ("traj", _p), ("sample_interval", C.c_int32), ("pad", C.c_int32),
+                ("inj_ia", _p), ("inj_ib", _p), ("inj_u", _p)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
@@ -57,6 +74,10 @@ def _load():
     lib.nce_ce_delta.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p, _p, C.c_int32, _p]
     lib.nce_pt_run_chunk.argtypes = [_p, _p, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _p]
     lib.nce_pt_exchange.argtypes = [_p, _p, C.c_int32, _p]
+    lib.nce_nn_model_create.argtypes = [_p, C.POINTER(NNModelDesc), C.POINTER(C.c_int32)]
+    lib.nce_nn_sizes.argtypes = [_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    lib.nce_nn_full_cache.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p, C.c_int32, _p]
+    lib.nce_nn_run_chunk.argtypes = [_p, _p, C.c_int32, C.c_int64, C.c_int32, _p]
     lib.nce_philox4x32_10.argtypes = [_p, _p, _p]
     lib.nce_philox4x32_10.restype = None
     return lib
@@ -65,7 +86,8 @@ def _load():
 lib = _load()
 
 EXPORTED = ("nce_last_error", "nce_ctx_create", "nce_ctx_destroy", "nce_ce_model_create", "nce_ce_energy",
-            "nce_ce_delta", "nce_pt_run_chunk", "nce_pt_exchange", "nce_philox4x32_10")
+            "nce_ce_delta", "nce_pt_run_chunk", "nce_pt_exchange", "nce_philox4x32_10", "nce_nn_model_create",
+            "nce_nn_sizes", "nce_nn_full_cache", "nce_nn_run_chunk")
 
 
 class NCEError(RuntimeError):
@@ -172,3 +194,53 @@ class Context:
     def pt_exchange(self, jobs, stream=None):
         arr = (ExchangeJob * len(jobs))(*jobs)
         check(lib.nce_pt_exchange(self.h, C.addressof(arr), len(jobs), stream_ptr(stream)))
+
+    # ── NeuralCE ──
+    def nn_model_create(self, geo, P, mu, sd, act="gate", readout="mean", de_scale=1.0):
+        """upload a NeuralCE model: geometry dict (gpu_layer0_implicit.build_geometry) and expanded parameters."""
+        layers = P["layers"]
+        keep = {}
+
+        def A(x, dt):
+            a = np.ascontiguousarray(x, dt); keep[id(a)] = a; return a
+        d = NNModelDesc()
+        for k in range(3):
+            d.d[k] = int(geo["d"][k])
+        d.n_cells = int(geo["n_cells"])
+        d.n_window = len(geo["window"]); d.window = A(geo["window"], np.int32).ctypes.data
+        d.n_entries = len(geo["ent_col"]); d.entry_slot = A(geo["ent_slot"], np.uint16).ctypes.data
+        d.entry_col = A(geo["ent_col"], np.int32).ctypes.data; d.entry_coef = A(geo["ent_coef"], np.float64).ctypes.data
+        d.n_shells = len(geo["shell_count"]); d.shell_count = A(geo["shell_count"], np.int32).ctypes.data
+        d.partners = A(geo["partners"], np.int32).ctypes.data if len(geo["partners"]) else None
+        d.c0 = len(P["lin"]); d.mu = A(mu, np.float64).ctypes.data; d.sd = A(sd, np.float64).ctypes.data
+        d.lin = A(P["lin"], np.float64).ctypes.data
+        d.n_layers = len(layers)
+        if layers:
+            d.layer_cin = A([L["self"].shape[0] for L in layers], np.int32).ctypes.data
+            d.layer_d = A([L["self"].shape[1] for L in layers], np.int32).ctypes.data
+            d.w_self = A(np.concatenate([np.ravel(L["self"]) for L in layers]), np.float64).ctypes.data
+            d.w_shell = A(np.concatenate([np.ravel(L["shell"]) for L in layers]), np.float64).ctypes.data
+            d.b = A(np.concatenate([np.ravel(L["b"]) for L in layers]), np.float64).ctypes.data
+            if "gate" in layers[0]:
+                d.w_gate = A(np.concatenate([np.ravel(L["gate"]) for L in layers]), np.float64).ctypes.data
+                d.gb = A(np.concatenate([np.ravel(L["gb"]) for L in layers]), np.float64).ctypes.data
+            d.head_w = A(P["head"]["w"], np.float64).ctypes.data
+        d.activation = {"gate": 0, "tanh": 0, "silu": 1}[act]
+        d.head_b = float(P["head"]["b"]); d.readout = {"mean": 0, "sum": 1}[readout]; d.de_scale = float(de_scale)
+        for l, cap in enumerate(geo["ball_cap"]):
+            d.ball_cap[l] = int(cap)
+        mid = C.c_int32()
+        check(lib.nce_nn_model_create(self.h, C.byref(d), C.byref(mid)))
+        ce, se = C.c_int64(), C.c_int64()
+        check(lib.nce_nn_sizes(self.h, mid, C.byref(ce), C.byref(se)))
+        return int(mid.value), int(ce.value), int(se.value)
+
+    def nn_full_cache(self, model, occ, cache, energy, precision=PREC_FP64, stream=None):
+        R, N = occ.shape
+        check(lib.nce_nn_full_cache(self.h, model, ptr(occ), R, N, ptr(cache), ptr(energy), int(precision),
+                                    stream_ptr(stream)))
+
+    def nn_run_chunk(self, jobs, n_steps, precision=PREC_MIXED, stream=None):
+        arr = (NNJob * len(jobs))(*jobs)
+        check(lib.nce_nn_run_chunk(self.h, C.addressof(arr), len(jobs), int(n_steps), int(precision),
+                                   stream_ptr(stream)))

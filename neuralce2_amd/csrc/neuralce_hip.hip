@@ -104,6 +104,37 @@ struct ModelDev {
     const float *fast_c32[4];
 };
 
+// NeuralCE model on the device (kernels in nn_update.hip)
+constexpr int NN_MAXL = 8;
+
+struct NNModelDev {
+    int32_t d0, d1, d2, nA;
+    int64_t n_cells, n_sites;
+    uint32_t div12_m, div12_s, div2_m, div2_s;
+    int32_t W0, J0, P, n_part, C0, L, act, readout, has_gate;
+    int32_t cin[NN_MAXL], dd[NN_MAXL];
+    int64_t off_self[NN_MAXL], off_shell[NN_MAXL], off_gate[NN_MAXL], off_b[NN_MAXL], off_gb[NN_MAXL];
+    const int4 *win;
+    const uint2 *ent_slot;       // four u16 slots, padding = 0xFFFF
+    const int32_t *ent_col;
+    const double *ent_coef;
+    const int32_t *shell_count;  // [P]
+    const int32_t *part_shell;   // [n_part] shell of each partner
+    const int4 *part;            // [n_part]
+    const double *mu, *sd, *lin, *wts, *head_w;
+    const float *mu_f, *sd_f, *lin_f, *wts_f, *head_w_f;
+    double head_b, de_scale;
+    int32_t ball_cap[NN_MAXL + 1];
+    int32_t hash_cap;            // power of two >= 2 * ball_cap[L]
+    int64_t off_h[NN_MAXL], off_e, cache_elems;
+    int64_t soff_h0, soff_src, soff_x, soff_h[NN_MAXL], soff_e, scratch_elems;
+    int32_t kmax;                // max c_in * (1 + P)
+};
+
+struct NNModelHost {
+    NNModelDev m;
+};
+
 struct nce_ctx {
     int device = 0;
     int n_cu = 0;
@@ -123,6 +154,9 @@ struct nce_ctx {
     int next_stage = 0;
     double *d_partial = nullptr;
     size_t partial_cap = 0;
+    std::vector<NNModelDev> nn_models;
+    void *d_nn_models = nullptr;
+    int cap_nn_models = 0;
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
@@ -993,6 +1027,7 @@ extern "C" void nce_ctx_destroy(nce_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->d_models) (void)hipFree(c->d_models);
     if (c->d_partial) (void)hipFree(c->d_partial);
+    if (c->d_nn_models) (void)hipFree(c->d_nn_models);
     for (auto &s : c->stage) {
         if (s.host) (void)hipHostFree(s.host);
         if (s.dev) (void)hipFree(s.dev);
@@ -1425,3 +1460,5 @@ extern "C" int nce_pt_exchange(nce_ctx *c, const nce_pt_exchange_job *jobs, int3
     HIP_TRY(hipGetLastError());
     return NCE_OK;
 }
+
+#include "nn_update.hip"

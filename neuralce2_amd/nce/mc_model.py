@@ -1,0 +1,109 @@
+"""NeuralCE MC machinery on the gfx950 kernels (reference: neuralce/nce/mc_model.py:60-228).
+
+``setup(mdl, atoms, seed_idx, dtype)`` returns the reference's keys (energy, full_cache, perm / inv_perm, sig0, ys, ym,
+n_atoms, propose, ...) backed by libneuralce_hip.so: the layer-0 template, shells and network are uploaded once, caches
+and local updates are HIP kernels.  Only the implicit (template-only) table mode exists; sites are in canonical order.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ..clusterspace import ClusterSpace
+from .gpu_layer0_implicit import build_geometry
+from .model_io import expand_params, load_model  # noqa: F401  (re-exported like the reference module)
+
+
+def silu_network(c0, width, depth, n_shells, seed=0, glorot=True):
+    """Parameters of the SiLU graph-convolution variant (Glorot-uniform weights from `seed`): per layer self and
+    shell maps, biases 0; per-site readout lin (layer 0) and w (last layer)."""
+    rng = np.random.default_rng(seed)
+
+    def glorot_u(fan_in, fan_out, shape):
+        a = np.sqrt(6.0 / (fan_in + fan_out))
+        return rng.uniform(-a, a, size=shape)
+    layers = []
+    cin = c0
+    for _ in range(depth):
+        layers.append(dict(self=glorot_u(cin, width, (cin, width)),
+                           shell=glorot_u(cin, width, (n_shells, cin, width)), b=np.zeros(width)))
+        cin = width
+    return dict(lin=glorot_u(c0, 1, (c0,)), layers=layers, head=dict(w=glorot_u(cin, 1, (cin,)), b=np.array(0.0)))
+
+
+class NNModel:
+    """A NeuralCE model uploaded for one cell: geometry, parameters, sizes."""
+
+    def __init__(self, cs, atoms, P, mu, sd, shell_edges, act="gate", readout="mean", ys=1.0, ym=0.0, device=0):
+        import torch
+        from .._lib import Context
+        if not torch.cuda.is_available():
+            raise RuntimeError("the NeuralCE MC path needs a ROCm GPU (libneuralce_hip.so); none is visible")
+        self.cs = cs
+        self.P, self.mu, self.sd = P, mu, sd
+        self.depth = len(P["layers"])
+        self.geo = build_geometry(cs, atoms, shell_edges, self.depth)
+        self.N = self.geo["N"]; self.nA = self.geo["nA"]
+        self.ys, self.ym = float(ys), float(ym)
+        self.readout = readout
+        self.device = torch.device("cuda", device)
+        self.ctx = Context.get(device)
+        de_scale = self.ys * self.N if readout == "mean" else 1.0
+        self.model, self.cache_elems, self.scratch_elems = self.ctx.nn_model_create(
+            self.geo, P, mu, sd, act=act, readout=readout, de_scale=de_scale)
+        self.de_scale = de_scale
+
+    def alloc(self, R, precision):
+        import torch
+        dt = torch.float64 if precision == 0 else torch.float32
+        cache = torch.empty((R, self.cache_elems), dtype=dt, device=self.device)
+        scratch = torch.empty((R, self.scratch_elems), dtype=dt, device=self.device)
+        return cache, scratch
+
+    def full_cache(self, occ, precision=1):
+        """occ (R, N) int8 device tensor -> (cache, energies in model units)."""
+        import torch
+        cache, _ = self.alloc(occ.shape[0], precision)
+        E = torch.empty(occ.shape[0], dtype=torch.float64, device=self.device)
+        self.ctx.nn_full_cache(self.model, occ, cache, E, precision)
+        return cache, E
+
+    def occ(self, sig):
+        import torch
+        s = torch.as_tensor(np.asarray(sig)).to(device=self.device, dtype=torch.int8)
+        if s.dim() == 1:
+            s = s[None]
+        return s.contiguous()
+
+
+def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0):
+    """Build the MC machinery for `atoms` (an ideal-lattice supercell of the model's primitive); see the module doc."""
+    if seed_idx is not None and int(seed_idx) < 0:
+        raise NotImplementedError("the seed-ensemble model is not part of the gfx950 path; pick one seed")
+    cs = ClusterSpace(mdl["primitive"], list(mdl["cutoffs"]), mdl["chemical_symbols"])
+    P, mu, sd = expand_params(mdl, seed_idx)
+    nn = NNModel(cs, atoms, P, mu, sd, list(mdl["shell_edges"]), act="gate", readout="mean",
+                 ys=mdl["ys"], ym=mdl["ym"], device=device)
+    prec = 0 if dtype in ("float64", None) else 1
+    geo = nn.geo
+
+    def energy(sig, spin_state=None):
+        """eV/atom, FP64 full recompute (the reference's gate path)."""
+        _, E = nn.full_cache(nn.occ(sig), precision=0)
+        return float(E[0].item()) * nn.ys + nn.ym
+
+    def full_cache(sig, spin_state=None):
+        return nn.full_cache(nn.occ(sig), precision=prec)
+
+    def local_delta_full(sig, flips):
+        """energy change (model units) of applying `flips` [[site, species], ...], from two full recomputes."""
+        s0 = np.asarray(sig).copy(); s1 = s0.copy()
+        for site, sp in np.asarray(flips):
+            s1[int(site)] = int(sp)
+        _, E = nn.full_cache(nn.occ(np.stack([s0, s1])), precision=0)
+        return float(E[1].item() - E[0].item())
+
+    return dict(cs=cs, P=P, mu=mu, sd=sd, depth=nn.depth, ys=nn.ys, ym=nn.ym, n_atoms=nn.N, n_active=nn.nA,
+                sig0=geo["sig"], energy=energy, full_cache=full_cache, local_delta_full=local_delta_full,
+                to_eV_per_atom=lambda E_std: E_std * nn.ys + nn.ym, dE_eV_total=lambda dE: dE * nn.ys * nn.N,
+                implicit=True, geo=geo, perm=geo["perm"], inv_perm=geo["inv_perm"], nn=nn, dtype=dtype,
+                precision=prec, cache_mode="h0")
