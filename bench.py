@@ -37,7 +37,67 @@ PUBLISHED_H100 = 4_605_830.0      # reference: CE attempts/s at N = 1,048,576, R
 CONFIGS = {
     "ce_1m": dict(n=64, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
     "ce_2k": dict(n=8, x=0.5, R=32, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="fp64"),
+    # NeuralCE: 3 SiLU graph-conv layers of width 64 over the 12 nearest neighbours on 40 cluster channels
+    "nce_16k": dict(n=16, x=0.5, R=64, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed", nce="silu"),
+    # NeuralCE: the reference's shipped PtNi model (depth 1, width 16, 3 shells) at 16,384 sites, 32 replicas
+    "nce_ptni_16k": dict(n=16, x=0.5, R=32, Tmin=700.0, Tmax=1100.0, exchange=5000, precision="mixed", nce="ptni"),
 }
+
+
+def nce_bench(args, cfg):
+    """NeuralCE local-update chunk kernel; roofline on MFMA-eligible FLOPs of the convolution layers."""
+    import torch
+    from neuralce2_amd.synthetic import fcc_cluster_space, fcc_cell
+    from neuralce2_amd.nce.mc_model import NNModel, silu_network
+    from neuralce2_amd.nce.mcmc import NNJobState
+    at = fcc_cell(cfg["n"], cfg["x"], args.seed)
+    if cfg["nce"] == "silu":
+        cs = fcc_cluster_space((9.0, 6.0, 5.0))
+        P = silu_network(len(cs), 64, 3, 1, seed=args.seed)
+        mu, sd = np.zeros(len(cs)), np.ones(len(cs))
+        nn = NNModel(cs, at, P, mu, sd, [0.0, 3.3], act="silu", readout="sum")
+    else:
+        from neuralce2_amd.nce.model_io import load_model, expand_params
+        from neuralce2_amd.clusterspace import ClusterSpace
+        mdl = load_model(os.path.join(ROOT, "tests", "golden", "nipt_v3_nomax_sel.pkl"))
+        cs = ClusterSpace(mdl["primitive"], mdl["cutoffs"], mdl["chemical_symbols"])
+        P, mu, sd = expand_params(mdl, 0)
+        nn = NNModel(cs, at, P, mu, sd, mdl["shell_edges"], ys=mdl["ys"], ym=mdl["ym"])
+    R = cfg["R"]
+    temps = np.geomspace(cfg["Tmin"], cfg["Tmax"], R)
+    betas = (1.0 / (8.617333262e-5 * temps)).astype(np.float32).astype(np.float64)
+    prec = 1 if cfg["precision"] == "mixed" else 0
+    sig = nn.geo["sig"].astype(np.int8)
+    st = NNJobState(nn, sig, betas, seed=args.seed, precision=prec)
+    chunk = args.chunk
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        st.run(chunk)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for c in range(args.steps):
+        evs[c][0].record(stream)
+        st.run(chunk)
+        evs[c][1].record(stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    k_ms = np.array([a.elapsed_time(b) for a, b in evs])
+    value = R * chunk * args.steps / dt
+    # convolution FLOPs per attempt: every layer on the union of the two flips' balls (mean union size measured
+    # on the host from the lattice offsets of random pairs), 2 FLOP per multiply-add of the self + shell maps
+    geo = nn.geo
+    fl = 0.0
+    sizes = geo["ball_sizes"]
+    for l, L in enumerate(P["layers"]):
+        cin, d = L["self"].shape
+        k = cin * (1 + len(geo["shell_count"])) * (2 if "gate" in L else 1)
+        fl += 2.0 * k * d * min(2 * sizes[l + 1], nn.N)
+    achieved = fl * R * chunk / (k_ms.mean() * 1e-3) / 1e12
+    peak = 157.3                                                   # FP32 vector (the kernel's FMA path)
+    acc = float(st.n_accept.sum().item()) / (R * chunk * (args.warmup + args.steps))
+    return dict(value=value, dt=dt, k_ms=k_ms, achieved=achieved, peak=peak, acc=acc, R=R, N=nn.N,
+                flops_attempt=fl, prec=prec)
 
 
 def parse():
@@ -90,6 +150,21 @@ def main():
     cfg = dict(CONFIGS[args.config])
     if args.precision:
         cfg["precision"] = args.precision
+    if "nce" in cfg:
+        res = nce_bench(args, cfg)
+        line = {"metric": "MC swap attempts/s (all replicas)", "value": res["value"], "unit": "attempts/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["dt"] / args.steps * 1e3,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f32" if res["prec"] else "f64", "data": "synthetic (seeded random occupations / weights)",
+                "config": {"workload": f"NeuralCE PT swap MC ({cfg['nce']}), fcc {cfg['n']}^3 cubic = {res['N']} "
+                                       f"sites, {res['R']} replicas", "chunk_steps": args.chunk},
+                "roofline": {"bound": "valu-fp32", "achieved": res["achieved"], "peak": res["peak"],
+                             "unit": "TFLOP/s", "frac": res["achieved"] / res["peak"], "traffic": None,
+                             "kernel": "nn_chunk_kernel", "kernel_ms_avg": float(res["k_ms"].mean()),
+                             "flops_per_attempt": res["flops_attempt"]},
+                "cpu_baseline": None, "acceptance": res["acc"]}
+        print(json.dumps(line), flush=True)
+        return
     prec = PREC_MIXED if cfg["precision"] == "mixed" else PREC_FP64
     ce = synthetic_ce(args.seed)
     at = fcc_cell(cfg["n"], cfg["x"], args.seed)
