@@ -37,6 +37,8 @@ PUBLISHED_H100 = 4_605_830.0      # reference: CE attempts/s at N = 1,048,576, R
 CONFIGS = {
     "ce_1m": dict(n=64, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
     "ce_2k": dict(n=8, x=0.5, R=32, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="fp64"),
+    "ce_131k": dict(n=32, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
+    "ce_10m": dict(n=136, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
     # NeuralCE: 3 SiLU graph-conv layers of width 64 over the 12 nearest neighbours on 40 cluster channels
     "nce_16k": dict(n=16, x=0.5, R=64, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed", nce="silu"),
     # NeuralCE: the reference's shipped PtNi model (depth 1, width 16, 3 shells) at 16,384 sites, 32 replicas
@@ -112,7 +114,21 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 3 device copies of 1 GiB (the known-size kernel that scales rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes)")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n, "launches": 3}), flush=True)
 
 
 def cpu_baseline(info, occ0, betas, target_s, seed):
@@ -142,6 +158,9 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is measured per GPU", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import torch
     from neuralce2_amd.synthetic import synthetic_ce, fcc_cell
     from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
@@ -222,7 +241,7 @@ def main():
                    "tile": args.tile},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "ce_chunk_kernel", "kernel_ms_avg": float(k_ms.mean()),
+                     "kernel": "ce_chunk_spec_kernel", "kernel_ms_avg": float(k_ms.mean()),
                      "bytes_per_attempt_alg": B_ALG_PER_ATTEMPT},
         "cpu_baseline": None,
         "acceptance": float(st.n_accept.sum().item()) / (R * chunk * (args.warmup + args.steps)),

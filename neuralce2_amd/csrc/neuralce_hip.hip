@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <utility>
 #include <cstdint>
 #include <cstdio>
@@ -94,14 +95,10 @@ struct ModelDev {
     const double *row_coef;     // [Rt]
     // division by d1*d2 and d2 of non-negative int32 values (Granlund-Montgomery: q = (hi + ((x - hi) >> 1)) >> sh)
     uint32_t div12_m, div12_s, div2_m, div2_s;
-    // binary fast path (one basis site, point functions -1 / +1, window <= 192 slots); index = log2(waves)
-    int32_t fast_nit[4];                // iterations of 64*waves entries, 0 = tile not available
-    int32_t fast_nit0;                  // 64-entry schedule: iterations on window word 0 only (the rest <= 4)
-    const uint8_t *fast_word[4];        // [nit] word (1 or 2) of the second mask, 0 = word 0 only
-    const unsigned long long *fast_m0[4];  // [nit][64*waves] bits of members in window word 0
-    const unsigned long long *fast_mk[4];  // [nit][64*waves] bits in window word fast_word[it]
-    const double *fast_c64[4];          // [nit][64*waves] coefficients (0 for padding)
-    const float *fast_c32[4];
+    // binary fast path (one basis site, point functions -1 / +1, window <= 192 slots): entry tables of the
+    // speculative kernel, 64 entries per iteration in three classes (FsLo / FsFull / FsTail), per precision
+    int32_t fs_ok, fs_nlo, fs_nfull, fs_ntail;
+    const unsigned char *fs_tab32, *fs_tab64;
 };
 
 // NeuralCE model on the device (kernels in nn_update.hip)
@@ -484,78 +481,72 @@ __global__ __launch_bounds__(64 * WAVES) void ce_chunk_kernel(const ModelDev *__
 }
 
 // ───────────────────────────────────────────── binary fast chunk kernel ──────────────────────────
-// Binary point functions (-1 / +1): a cluster's product is (-1)^(number of members whose species is 0), i.e. the
-// parity of the AND of the window's spin bits with the cluster's member mask.  Away from each other and from their
-// own periodic images, swapping two different species flips exactly the centre of each window, so
-//   dE = -2 * factor * (h_a + h_b),   h_x = sum over the entries of x of coef * (-1)^parity(bits_x & mask).
-// Windows are gathered D steps ahead of use (Philox proposals are independent of the state); at use, each thread
-// patches its prefetched slots with the commits of the D steps in between.  Steps whose windows contain the other
-// site or an image of their own centre take the general value-table path.
+// Binary point functions (-1 / +1): a cluster's product is (-1)^(number of members whose species is 0).  Away from
+// each other and from their own periodic images, swapping two different species flips exactly the centre of each
+// window, so dE = -2 * factor * (h_a + h_b) with h_x = sum over the entries e of x of coef_e * prod_e.  Splitting off
+// the centre, prod_e = (-1)^[centre is species 0] * (-1)^parity(bits_x & mask_e), mask_e = the other members' window
+// bits; and since the two centres hold different species, h_a + h_b = (-1)^[s_a = 0] * (S_a - S_b) with
+// S_x = sum_e coef_e * (-1)^parity(bits_x & mask_e).  Bit 0 of window word 0 (the centre) is set to 1 in bits_x
+// and to [coef_e < 0] in mask_e, so the table stores |coef_e| and a term is one AND + popcount + shift-or of the
+// parity into the sign bit.  Steps whose windows contain the other site or an image of their own centre take the
+// value-table path.
 template <typename Real>
-__device__ inline Real signed_coef(Real c, uint32_t parity);
+struct FsLo;                                   // every member in window bits 0..31
 template <>
-__device__ inline float signed_coef<float>(float c, uint32_t p) {
-    return __int_as_float(__float_as_int(c) ^ (int)(p << 31));
-}
-template <>
-__device__ inline double signed_coef<double>(double c, uint32_t p) {
-    return __longlong_as_double(__double_as_longlong(c) ^ ((long long)p << 63));
-}
-
-struct FastLds {
-    size_t slot, coef, valA, valB, red, bits, ctr, near, cm, total;
+struct FsLo<float> {
+    uint32_t m, c;
 };
-
+template <>
+struct FsLo<double> {
+    uint32_t m, pad;
+    uint64_t c;
+};
+struct FsFull {                                // members in window word 0
+    uint64_t m, c;                             // c: |coef| bits (float in the low half)
+};
+struct FsTail {                                // a member in window word `word` (1 or 2)
+    uint64_t m0, mk, c;
+    uint32_t word, pad;
+};
 template <typename Real>
-__host__ __device__ inline FastLds fast_lds_layout(int J, int Wcap, int waves) {
-    FastLds L;
-    size_t o = 0;
-    L.slot = o; o = align16(o + (size_t)J * sizeof(uint2));
-    L.coef = o; o = align16(o + (size_t)J * sizeof(Real));
-    L.valA = o; o = align16(o + (size_t)Wcap * 2 * sizeof(Real));
-    L.valB = o; o = align16(o + (size_t)Wcap * 2 * sizeof(Real));
-    L.red = o;  o = align16(o + (size_t)waves * sizeof(double));
-    L.bits = o; o = align16(o + 2 * 6 * sizeof(unsigned long long));
-    L.ctr = o;  o = align16(o + 2 * 2 * sizeof(int));
-    L.near = o; o = align16(o + 2 * (size_t)waves * sizeof(int));
-    L.cm = o;   o = align16(o + 4 * 16 * sizeof(int4));               // commit ring, up to 4 * 16 steps
-    L.total = o;
-    return L;
+__host__ __device__ inline size_t fs_bytes(int nlo, int nfull, int ntail) {
+    return (size_t)64 * ((size_t)nlo * sizeof(FsLo<Real>) + (size_t)nfull * sizeof(FsFull) +
+                         (size_t)ntail * sizeof(FsTail));
 }
 
-template <int WAVES>
-struct WaveIdx {
-    static constexpr int value = WAVES == 1 ? 0 : WAVES == 2 ? 1 : WAVES == 4 ? 2 : 3;
-};
+// |c| (sign bit clear) with the low bit of n as its sign
+__device__ inline float fs_term(uint32_t c, uint32_t n) { return __int_as_float((int)((n << 31) | c)); }
+__device__ inline double fs_term(uint64_t c, uint32_t n) {
+    return __longlong_as_double((long long)(((uint64_t)(n << 31) << 32) | c));
+}
 
 // Speculative parallel steps: wave w of a W-wave workgroup evaluates step t0 + w of the same replica from the
 // current occupations; the window loads of all W steps are in flight together.  A uniform pass then applies the
 // steps in order (Metropolis, running total, statistics); a step whose windows contain a site changed by an
 // earlier accepted step of the same group is re-evaluated on patched species before its decision, so the result is
 // exactly the sequential chain.
-constexpr int SPEC_MAXIT = 36;       // 64-entry iterations with one window word (+ up to 4 with a second word)
-
 struct SpecRec {
     double dE;
+    double lu;                       // log of the step's Metropolis uniform, in the precision of the decision
     int sa, sb, valid, conflict;
 };
 
 template <typename Real>
 struct SpecLds {
-    size_t slot, coef, val, prop, u, rec, commit, total;
+    size_t slot, coef, val, prop, rec, commit, fs, total;
 };
 
 template <typename Real>
-__host__ __device__ inline SpecLds<Real> spec_lds_layout(int J, int Wcap, int waves) {
+__host__ __device__ inline SpecLds<Real> spec_lds_layout(int J, int Wcap, int waves, int nlo, int nfull, int ntail) {
     SpecLds<Real> L;
     size_t o = 0;
     L.slot = o;   o = align16(o + (size_t)J * sizeof(uint2));
     L.coef = o;   o = align16(o + (size_t)J * sizeof(Real));
     L.val = o;    o = align16(o + (size_t)waves * 2 * Wcap * 2 * sizeof(Real));
     L.prop = o;   o = align16(o + (size_t)waves * sizeof(int2));
-    L.u = o;      o = align16(o + (size_t)waves * sizeof(double));
     L.rec = o;    o = align16(o + (size_t)waves * sizeof(SpecRec));
     L.commit = o; o = align16(o + (size_t)waves * sizeof(int4));
+    L.fs = o;     o = align16(o + fs_bytes<Real>(nlo, nfull, ntail));
     L.total = o;
     return L;
 }
@@ -574,52 +565,32 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int J = mg.J, Wcap = mg.Wcap, Wn = mg.W_b[0], Jb0 = mg.J_b[0];
-    const SpecLds<Real> L = spec_lds_layout<Real>(J, Wcap, W);
+    const int nlo = mg.fs_nlo, nfull = mg.fs_nfull, ntail = mg.fs_ntail;
+    const SpecLds<Real> L = spec_lds_layout<Real>(J, Wcap, W, nlo, nfull, ntail);
     uint2 *s_slot = (uint2 *)(smem + L.slot);
     Real *s_coef = (Real *)(smem + L.coef);
     Real2<Real> *valA = (Real2<Real> *)(smem + L.val) + (size_t)w * 2 * Wcap;
     Real2<Real> *valB = valA + Wcap;
     int2 *s_prop = (int2 *)(smem + L.prop);
-    double *s_u = (double *)(smem + L.u);
     SpecRec *s_rec = (SpecRec *)(smem + L.rec);
     int4 *s_commit = (int4 *)(smem + L.commit);
+    const FsLo<Real> *s_lo = (const FsLo<Real> *)(smem + L.fs);
+    const FsFull *s_full = (const FsFull *)(s_lo + 64 * nlo);
+    const FsTail *s_tail = (const FsTail *)(s_full + 64 * nfull);
     for (int i = tid; i < J; i += BS) {
         s_slot[i] = mg.ent_slot[i];
         if constexpr (sizeof(Real) == 8) s_coef[i] = mg.ent_coef[i];
         else s_coef[i] = mg.ent_coef_f[i];
     }
+    {
+        const uint4 *src = (const uint4 *)(sizeof(Real) == 8 ? mg.fs_tab64 : mg.fs_tab32);
+        uint4 *dst = (uint4 *)(smem + L.fs);
+        const int n16 = (int)(fs_bytes<Real>(nlo, nfull, ntail) / 16);
+        for (int i = tid; i < n16; i += BS) dst[i] = src[i];
+    }
     if (lane == 0) {
         valA[Wcap - 1].o = valA[Wcap - 1].n = (Real)1;
         valB[Wcap - 1].o = valB[Wcap - 1].n = (Real)1;
-    }
-    // entry masks of this lane (64-entry schedule): nit0 iterations on window word 0, then ntail with a second word
-    const int nit0 = mg.fast_nit0, ntail = mg.fast_nit[0] - mg.fast_nit0;
-    unsigned long long m0[SPEC_MAXIT], m0t[4], mkt[4];
-    Real cf[SPEC_MAXIT], cft[4];
-    int kwt[4];
-    {
-        const NCE_GLOBAL unsigned long long *gm0 = gptr(mg.fast_m0[0]), *gmk = gptr(mg.fast_mk[0]);
-#pragma unroll
-        for (int i = 0; i < SPEC_MAXIT; ++i) {
-            m0[i] = 0; cf[i] = (Real)0;
-            if (i < nit0) {
-                m0[i] = gm0[i * 64 + lane];
-                if constexpr (sizeof(Real) == 8) cf[i] = gptr(mg.fast_c64[0])[i * 64 + lane];
-                else cf[i] = gptr(mg.fast_c32[0])[i * 64 + lane];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            m0t[i] = 0; mkt[i] = 0; cft[i] = (Real)0; kwt[i] = 0;
-            if (i < ntail) {
-                const int ii = nit0 + i;
-                m0t[i] = gm0[ii * 64 + lane];
-                mkt[i] = gmk[ii * 64 + lane];
-                if constexpr (sizeof(Real) == 8) cft[i] = gptr(mg.fast_c64[0])[ii * 64 + lane];
-                else cft[i] = gptr(mg.fast_c32[0])[ii * 64 + lane];
-                kwt[i] = gptr(mg.fast_word[0])[ii];
-            }
-        }
     }
     int4 wofs[NS];
     bool wok[NS];
@@ -657,6 +628,9 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     double s1 = 0.0, s2 = 0.0;
     int64_t nacc = 0;
     const int64_t inj_base = (int64_t)r * n_steps;
+    const bool accumulate = job.accumulate != 0;
+    const int64_t sample_interval = job.sample_interval, n_rep = job.n_replicas;
+    int64_t next_sample = sample_interval, n_sampled = 0;
 
     // evaluation of this wave's step from (patched) species in registers
     auto evaluate = [&](int ia, int ib, const int (&sA)[NS], const int (&sB)[NS], const int (&pA)[NS],
@@ -669,23 +643,31 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                 WA[j] = __ballot(wok[j] && pA[j] == 0);
                 WB[j] = __ballot(wok[j] && pB[j] == 0);
             }
-            Real acc = (Real)0;
-#pragma unroll
-            for (int i = 0; i < SPEC_MAXIT; ++i) {
-                if (i < nit0) {
-                    acc += signed_coef<Real>(cf[i], (uint32_t)__popcll(WA[0] & m0[i]) & 1u);
-                    acc += signed_coef<Real>(cf[i], (uint32_t)__popcll(WB[0] & m0[i]) & 1u);
-                }
+            WA[0] |= 1ull;                             // the sign bit of every term (see above)
+            WB[0] |= 1ull;
+            using CT = typename std::conditional<sizeof(Real) == 8, uint64_t, uint32_t>::type;
+            Real accA = (Real)0, accB = (Real)0;
+            const uint32_t a32 = (uint32_t)WA[0], b32 = (uint32_t)WB[0];
+#pragma unroll 4
+            for (int i = lane; i < 64 * nlo; i += 64) {
+                const FsLo<Real> e = s_lo[i];
+                accA += fs_term((CT)e.c, (uint32_t)__builtin_popcount(a32 & e.m));
+                accB += fs_term((CT)e.c, (uint32_t)__builtin_popcount(b32 & e.m));
             }
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (i < ntail) {
-                    const unsigned long long a1 = kwt[i] == 1 ? WA[1] : WA[2], b1 = kwt[i] == 1 ? WB[1] : WB[2];
-                    acc += signed_coef<Real>(cft[i], (uint32_t)__popcll((WA[0] & m0t[i]) ^ (a1 & mkt[i])) & 1u);
-                    acc += signed_coef<Real>(cft[i], (uint32_t)__popcll((WB[0] & m0t[i]) ^ (b1 & mkt[i])) & 1u);
-                }
+#pragma unroll 4
+            for (int i = lane; i < 64 * nfull; i += 64) {
+                const FsFull e = s_full[i];
+                accA += fs_term((CT)e.c, (uint32_t)__popcll(WA[0] & e.m));
+                accB += fs_term((CT)e.c, (uint32_t)__popcll(WB[0] & e.m));
             }
-            sum = (Real)-2 * wave_sum(acc);
+            for (int i = lane; i < 64 * ntail; i += 64) {
+                const FsTail e = s_tail[i];
+                const unsigned long long a1 = e.word == 1 ? WA[1] : WA[2], b1 = e.word == 1 ? WB[1] : WB[2];
+                accA += fs_term((CT)e.c, (uint32_t)__popcll((WA[0] & e.m0) ^ (a1 & e.mk)));
+                accB += fs_term((CT)e.c, (uint32_t)__popcll((WB[0] & e.m0) ^ (b1 & e.mk)));
+            }
+            const Real S = wave_sum(accA - accB);
+            sum = sa == 0 ? (Real)2 * S : (Real)-2 * S;
         } else {
 #pragma unroll
             for (int j = 0; j < NS; ++j) {
@@ -754,7 +736,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             pA[j] = wok[j] ? (int)occ[sA[j]] : 1;
             pB[j] = wok[j] ? (int)occ[sB[j]] : 1;
         }
-        if (lane == 0) { s_prop[w] = make_int2(ia, ib); s_u[w] = u; }
+        if (lane == 0) s_prop[w] = make_int2(ia, ib);
         __syncthreads();
         // 2. conflicts with the earlier steps of the group, centres, near flag, energy change
         int conflict = 0;
@@ -773,19 +755,24 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         const bool near = __ballot(nr) != 0ull;
         const int sa = __builtin_amdgcn_readfirstlane(pA[0]), sb = __builtin_amdgcn_readfirstlane(pB[0]);
         const bool valid = active && sa != sb;
-        double dE = 0.0;
+        double dE = 0.0, lu = 0.0;
         if (valid) dE = evaluate(ia, ib, sA, sB, pA, pB, sa, sb, near);
+        // also for a null proposal: a re-evaluation after an earlier commit can make it a real swap
+        if (active) {
+            if constexpr (sizeof(Real) == 8) lu = log(u);
+            else lu = (double)logf((float)u);
+        }
         if (lane == 0) {
             SpecRec rec;
-            rec.dE = dE; rec.sa = sa; rec.sb = sb; rec.valid = valid; rec.conflict = conflict;
+            rec.dE = dE; rec.lu = lu; rec.sa = sa; rec.sb = sb; rec.valid = valid; rec.conflict = conflict;
             s_rec[w] = rec;
         }
         __syncthreads();
-        // 3. decisions in step order (every wave, same values)
+        // 3. decisions in step order: every wave takes the same accept decisions (they drive the re-evaluation
+        // barriers); wave 0 alone keeps the running total, the statistics and the trajectory
         int acc_mask = 0, n_commit = 0;
-        for (int i = 0; i < W; ++i) {
-            const int64_t ti = t0 + i;
-            if (ti >= n_steps) break;
+        const int nW = (int)((n_steps - t0) < (int64_t)W ? (n_steps - t0) : (int64_t)W);
+        for (int i = 0; i < nW; ++i) {
             SpecRec rec = s_rec[i];
             if (rec.conflict & acc_mask) {
                 // re-evaluate step i on the species after the accepted steps before it (wave i); the barrier makes
@@ -810,41 +797,47 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                     double dE2 = 0.0;
                     if (valid2) dE2 = evaluate(ia, ib, sA, sB, qA, qB, sa2, sb2, near);
                     if (lane == 0) {
-                        SpecRec r2;
-                        r2.dE = dE2; r2.sa = sa2; r2.sb = sb2; r2.valid = valid2; r2.conflict = rec.conflict;
+                        SpecRec r2 = rec;
+                        r2.dE = dE2; r2.sa = sa2; r2.sb = sb2; r2.valid = valid2;
                         s_rec[i] = r2;
                     }
                 }
                 __syncthreads();
                 rec = s_rec[i];
             }
+            bool accept = false;
             if (rec.valid) {
-                const double ui = s_u[i];
-                bool accept;
-                if constexpr (sizeof(Real) == 8) accept = log(ui) < -beta * rec.dE;
-                else accept = logf((float)ui) < -(beta_f * (float)rec.dE);
-                if (accept) {
+                if constexpr (sizeof(Real) == 8) accept = rec.lu < -beta * rec.dE;
+                else accept = (float)rec.lu < -(beta_f * (float)rec.dE);
+            }
+            if (accept) {
+                acc_mask |= 1 << i;
+                if (tid == 0) {
                     const int2 pi = s_prop[i];
+                    s_commit[n_commit] = make_int4(pi.x, pi.y, rec.sb, rec.sa);
+                    occ[pi.x] = (int8_t)rec.sb;
+                    occ[pi.y] = (int8_t)rec.sa;
+                }
+                ++n_commit;
+                // the commit list is read by a later re-evaluation only after that step's barrier
+            }
+            if (w == 0) {
+                if (accept) {
                     E += rec.dE;
                     ++nacc;
-                    acc_mask |= 1 << i;
-                    if (tid == 0) {
-                        s_commit[n_commit] = make_int4(pi.x, pi.y, rec.sb, rec.sa);
-                        occ[pi.x] = (int8_t)rec.sb;
-                        occ[pi.y] = (int8_t)rec.sa;
-                    }
-                    ++n_commit;
-                    // the commit list is read by a later re-evaluation only after that step's barrier
+                }
+                emin = fmin(emin, E);
+                if (accumulate) {
+                    const double x = E - eref;
+                    s1 += x;
+                    s2 += x * x;
+                }
+                if (traj && t0 + i + 1 == next_sample) {
+                    if (lane == 0) traj[n_sampled * n_rep + slot_r] = E;
+                    next_sample += sample_interval;
+                    ++n_sampled;
                 }
             }
-            emin = fmin(emin, E);
-            if (job.accumulate) {
-                const double x = E - eref;
-                s1 += x;
-                s2 += x * x;
-            }
-            if (traj && ((ti + 1) % job.sample_interval) == 0 && tid == 0)
-                traj[((ti + 1) / job.sample_interval - 1) * job.n_replicas + slot_r] = E;
         }
         __syncthreads();
     }
@@ -1067,8 +1060,6 @@ static int sync_models(nce_ctx *c) {
     return NCE_OK;
 }
 
-static const int FAST_MAXIT[4] = {40, 24, 12, 8};
-
 // magic numbers for unsigned division by d (1 <= d < 2^31) valid for every 32-bit dividend
 static void magic_u32(uint32_t d, uint32_t *m, uint32_t *sh) {
     uint32_t l = 0;
@@ -1081,67 +1072,80 @@ static void magic_u32(uint32_t d, uint32_t *m, uint32_t *sh) {
 
 // Fast-path tables: entries scheduled in iterations of 64*waves, one second window word per iteration.
 static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m) {
-    for (int w = 0; w < 4; ++w) m.fast_nit[w] = 0;
+    m.fs_ok = 0;
+    m.fs_nlo = m.fs_nfull = m.fs_ntail = 0;
+    m.fs_tab32 = m.fs_tab64 = nullptr;
     if (d->n_basis != 1 || d->n_active != 1 || d->active_nbasis[0] != 1 || d->max_window > 192) return NCE_OK;
     for (int sl = 0; sl < d->n_sublattices; ++sl) {
         const double *pf = d->pf + sl * 4;
         if (pf[0] != 1.0 || pf[1] != 1.0 || pf[2] != -1.0 || pf[3] != 1.0) return NCE_OK;
     }
+    // classify the entries of the one basis site (entries with a zero coefficient contribute nothing)
+    struct Ent {
+        unsigned long long m0, mk;
+        double c;
+        int word;
+    };
+    std::vector<Ent> lo, full, tail;
     const int Jb = d->entries_per_basis[0];
-    std::vector<int> grp[3];
-    std::vector<unsigned long long> A(Jb), B(Jb);
     for (int j = 0; j < Jb; ++j) {
-        unsigned long long a = 0, b = 0;
-        int k = 0, centre = 0;
+        Ent e{0ull, 0ull, d->entry_coef[j], 0};
+        int centre = 0;
         for (int q = 0; q < 4; ++q) {
             const uint32_t sl = d->entry_slot[(size_t)j * 4 + q];
             if (sl == 0xFFFFu) continue;
-            if (sl == 0) ++centre;
+            if (sl == 0) { ++centre; continue; }
             const int word = (int)(sl >> 6);
             const unsigned long long bit = 1ull << (sl & 63);
-            if (word == 0) a |= bit;
+            if (word == 0) e.m0 ^= bit;
             else {
-                if (k && k != word) return NCE_OK;          // members in two upper words: general path only
-                k = word;
-                b |= bit;
+                if (e.word && e.word != word) return NCE_OK;   // members in two upper words: general path only
+                e.word = word;
+                e.mk ^= bit;
             }
         }
         if (centre != 1) return NCE_OK;
-        A[j] = a; B[j] = b;
-        grp[k].push_back(j);
+        if (e.c == 0.0) continue;
+        if (e.c < 0.0) { e.m0 |= 1ull; e.c = -e.c; }
+        if (e.word) tail.push_back(e);
+        else if ((e.m0 >> 32) == 0) lo.push_back(e);
+        else full.push_back(e);
     }
-    for (int w = 0; w < 4; ++w) {
-        const int BS = 64 << w;
-        int nit = 0;
-        for (int k = 0; k < 3; ++k) nit += (int)((grp[k].size() + BS - 1) / BS);
-        if (nit < 1 || nit > FAST_MAXIT[w]) continue;
-        std::vector<uint8_t> word(nit);
-        std::vector<unsigned long long> m0((size_t)nit * BS, 0ull), mk((size_t)nit * BS, 0ull);
-        std::vector<double> c64((size_t)nit * BS, 0.0);
-        std::vector<float> c32((size_t)nit * BS, 0.0f);
-        int itn = 0;
-        for (int k = 0; k < 3; ++k) {
-            for (size_t base = 0; base < grp[k].size(); base += BS, ++itn) {
-                word[itn] = (uint8_t)k;
-                for (int t = 0; t < BS && base + t < grp[k].size(); ++t) {
-                    const int j = grp[k][base + t];
-                    const size_t idx = (size_t)itn * BS + t;
-                    m0[idx] = A[j]; mk[idx] = B[j];
-                    c64[idx] = d->entry_coef[j]; c32[idx] = (float)d->entry_coef[j];
-                }
-            }
+    const int nlo = (int)((lo.size() + 63) / 64), nfull = (int)((full.size() + 63) / 64),
+              ntail = (int)((tail.size() + 63) / 64);
+    std::vector<unsigned char> t32(fs_bytes<float>(nlo, nfull, ntail), 0), t64(fs_bytes<double>(nlo, nfull, ntail), 0);
+    auto bits32 = [](double v) { float f = (float)v; uint32_t u; std::memcpy(&u, &f, 4); return u; };
+    auto bits64 = [](double v) { uint64_t u; std::memcpy(&u, &v, 8); return u; };
+    {
+        FsLo<float> *l32 = (FsLo<float> *)t32.data();
+        FsLo<double> *l64 = (FsLo<double> *)t64.data();
+        for (size_t i = 0; i < lo.size(); ++i) {
+            l32[i].m = l64[i].m = (uint32_t)lo[i].m0;
+            l32[i].c = bits32(lo[i].c);
+            l64[i].c = bits64(lo[i].c);
         }
-        if (w == 0) {
-            m.fast_nit0 = (int)((grp[0].size() + BS - 1) / BS);
-            if (nit - m.fast_nit0 > 4 || m.fast_nit0 > SPEC_MAXIT) continue;
+        FsFull *f32 = (FsFull *)(l32 + 64 * nlo), *f64 = (FsFull *)(l64 + 64 * nlo);
+        for (size_t i = 0; i < full.size(); ++i) {
+            f32[i].m = f64[i].m = full[i].m0;
+            f32[i].c = bits32(full[i].c);
+            f64[i].c = bits64(full[i].c);
         }
-        UP(word.data(), word.size(), &m.fast_word[w]);
-        UP(m0.data(), m0.size(), &m.fast_m0[w]);
-        UP(mk.data(), mk.size(), &m.fast_mk[w]);
-        UP(c64.data(), c64.size(), &m.fast_c64[w]);
-        UP(c32.data(), c32.size(), &m.fast_c32[w]);
-        m.fast_nit[w] = nit;
+        FsTail *u32 = (FsTail *)(f32 + 64 * nfull), *u64 = (FsTail *)(f64 + 64 * nfull);
+        for (size_t i = 0; i < tail.size(); ++i) {
+            u32[i].m0 = u64[i].m0 = tail[i].m0;
+            u32[i].mk = u64[i].mk = tail[i].mk;
+            u32[i].word = u64[i].word = (uint32_t)tail[i].word;
+            u32[i].c = bits32(tail[i].c);
+            u64[i].c = bits64(tail[i].c);
+        }
+        for (size_t i = tail.size(); i < (size_t)64 * ntail; ++i) u32[i].word = u64[i].word = 1;
     }
+    if (!t32.empty()) {
+        UP(t32.data(), t32.size(), &m.fs_tab32);
+        UP(t64.data(), t64.size(), &m.fs_tab64);
+    }
+    m.fs_nlo = nlo; m.fs_nfull = nfull; m.fs_ntail = ntail;
+    m.fs_ok = 1;
     return NCE_OK;
 }
 
@@ -1382,7 +1386,7 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
     const int per_cu = (int)((items.size() + c->n_cu - 1) / c->n_cu);
     // general kernel: waves cooperating on one step of a replica; speculative kernel: steps evaluated in parallel
     int W = tile == 0 ? (per_cu >= 4 ? 1 : per_cu >= 2 ? 2 : 4) : (tile > 8 ? 8 : tile);
-    const int tile_spec = (tile == 4 || tile == 8) ? tile : tile == 16 ? 8 : (per_cu >= 4 ? 4 : 8);
+    const int tile_spec = (tile == 4 || tile == 8 || tile == 16) ? tile : (per_cu >= 4 ? 4 : per_cu >= 2 ? 8 : 16);
     // job table and work items in one staged buffer
     const size_t jb_bytes = align16(sizeof(nce_pt_job) * n_jobs);
     std::vector<unsigned char> buf(jb_bytes + items.size() * sizeof(int2));
@@ -1406,24 +1410,36 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
     const bool inj = jobs[0].inj_ia != nullptr;
     size_t lds_fast = 0;
     int SW = tile_spec;
+    auto spec_lds = [&](int waves) {
+        size_t need = 0;
+        for (int j = 0; j < n_jobs; ++j) {
+            const ModelDev &m = c->models[jobs[j].model];
+            need = std::max(need, precision == NCE_PREC_FP64
+                                      ? spec_lds_layout<double>(m.J, m.Wcap, waves, m.fs_nlo, m.fs_nfull, m.fs_ntail).total
+                                      : spec_lds_layout<float>(m.J, m.Wcap, waves, m.fs_nlo, m.fs_nfull, m.fs_ntail).total);
+        }
+        return need;
+    };
     for (int j = 0; j < n_jobs; ++j) {
         const ModelDev &m = c->models[jobs[j].model];
-        if (m.fast_nit[0] == 0 || m.n_sites >= ((int64_t)1 << 31)) fast = false;
+        if (!m.fs_ok || m.n_sites >= ((int64_t)1 << 31)) fast = false;
         if ((jobs[j].inj_ia != nullptr) != inj) fast = false;          // one stream kind per launch
-        lds_fast = std::max(lds_fast, precision == NCE_PREC_FP64 ? spec_lds_layout<double>(m.J, m.Wcap, SW).total
-                                                                 : spec_lds_layout<float>(m.J, m.Wcap, SW).total);
     }
+    if (SW == 16 && spec_lds(16) > 160 * 1024) SW = 8;
+    lds_fast = spec_lds(SW);
     const char *fg = std::getenv("NCE_FORCE_GENERAL");               // tests: run the general kernel on any model
     if (fg && fg[0] == '1') fast = false;
     if (fast && lds_fast <= 160 * 1024) {
         if (precision == NCE_PREC_FP64) {
             switch (SW) {
                 case 4: return launch_chunk_spec<double, 4>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
+                case 16: return launch_chunk_spec<double, 16>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
                 default: return launch_chunk_spec<double, 8>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
             }
         }
         switch (SW) {
             case 4: return launch_chunk_spec<float, 4>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
+            case 16: return launch_chunk_spec<float, 16>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
             default: return launch_chunk_spec<float, 8>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
         }
     }

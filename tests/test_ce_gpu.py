@@ -83,7 +83,8 @@ def _near_tie(o, occ_before_path, *a):
 
 CHUNK_CASES = [(4, 8, 400, 0, 0), (8, 32, 300, 0, 0), (2, 4, 500, 0, 0), (4, 8, 400, 1, 0),
                (8, 8, 301, 0, 4), (8, 8, 302, 0, 8), (8, 8, 303, 1, 4), (8, 8, 305, 1, 8),
-               (16, 6, 400, 0, 4), (16, 6, 400, 1, 8), (1, 4, 203, 0, 8), (2, 4, 203, 1, 4)]
+               (16, 6, 400, 0, 4), (16, 6, 400, 1, 8), (1, 4, 203, 0, 8), (2, 4, 203, 1, 4),
+               (8, 8, 403, 1, 16), (4, 6, 301, 0, 16), (16, 4, 500, 1, 16)]
 
 
 @pytest.mark.parametrize("n,R,steps,precision,tile", CHUNK_CASES)
@@ -125,7 +126,7 @@ def test_chunk_injected_streams_bitexact(torch_dev, ptni_ce, ptni_template, n, R
         assert np.max(np.abs(st.energy.cpu().numpy() - E_full)) / N < 1e-9 * (1 if precision == 0 else 1e3)
 
 
-@pytest.mark.parametrize("n,tile", [(4, 0), (8, 8), (8, 4), (16, 4), (16, 8)])
+@pytest.mark.parametrize("n,tile", [(4, 0), (8, 8), (8, 4), (16, 4), (16, 8), (8, 16)])
 def test_chunk_philox_stream_matches_oracle_replay(torch_dev, ptni_ce, ptni_template, n, tile):
     """the kernel's own Philox proposals replayed by the oracle give the same trajectory."""
     import torch
