@@ -768,63 +768,89 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             s_rec[w] = rec;
         }
         __syncthreads();
-        // 3. decisions in step order: every wave takes the same accept decisions (they drive the re-evaluation
-        // barriers); wave 0 alone keeps the running total, the statistics and the trajectory
-        int acc_mask = 0, n_commit = 0;
+        // 3. decisions.  Lane i of every wave holds step i's record; the tentative decisions are taken in parallel,
+        // then every step whose windows hold a site changed by an accepted earlier step of the group is
+        // re-evaluated on the patched species by its own wave, in step order (all waves follow the same uniform
+        // sequence of barriers).
         const int nW = (int)((n_steps - t0) < (int64_t)W ? (n_steps - t0) : (int64_t)W);
-        for (int i = 0; i < nW; ++i) {
-            SpecRec rec = s_rec[i];
-            if (rec.conflict & acc_mask) {
-                // re-evaluate step i on the species after the accepted steps before it (wave i); the barrier makes
-                // the commit list written by thread 0 visible
-                __syncthreads();
-                if (w == i) {
-                    int qA[NS], qB[NS];
-#pragma unroll
-                    for (int j = 0; j < NS; ++j) { qA[j] = pA[j]; qB[j] = pB[j]; }
-                    for (int c = 0; c < n_commit; ++c) {
-                        const int4 cm = s_commit[c];
-#pragma unroll
-                        for (int j = 0; j < NS; ++j) {
-                            if (sA[j] == cm.x) qA[j] = cm.z;
-                            if (sA[j] == cm.y) qA[j] = cm.w;
-                            if (sB[j] == cm.x) qB[j] = cm.z;
-                            if (sB[j] == cm.y) qB[j] = cm.w;
-                        }
+        SpecRec ri = s_rec[lane < nW ? lane : nW - 1];
+        auto accepts = [&](const SpecRec &q) __attribute__((always_inline)) -> bool {
+            if (!q.valid) return false;
+            if constexpr (sizeof(Real) == 8) return q.lu < -beta * q.dE;
+            else return (float)q.lu < -(beta_f * (float)q.dE);
+        };
+        uint64_t accm = __ballot(lane < nW && accepts(ri));
+        uint64_t need = __ballot(lane < nW && (ri.conflict & (int)accm) != 0);
+        while (need != 0ull) {
+            const int f = __builtin_ctzll(need);
+            if (tid == 0) {                            // commit list of the accepted steps before f, in order
+                int n = 0;
+                for (int i = 0; i < f; ++i)
+                    if ((accm >> i) & 1ull) {
+                        const int2 pi = s_prop[i];
+                        const SpecRec q = s_rec[i];
+                        s_commit[n++] = make_int4(pi.x, pi.y, q.sb, q.sa);
                     }
-                    const int sa2 = __builtin_amdgcn_readfirstlane(qA[0]), sb2 = __builtin_amdgcn_readfirstlane(qB[0]);
-                    const bool valid2 = sa2 != sb2;
-                    double dE2 = 0.0;
-                    if (valid2) dE2 = evaluate(ia, ib, sA, sB, qA, qB, sa2, sb2, near);
-                    if (lane == 0) {
-                        SpecRec r2 = rec;
-                        r2.dE = dE2; r2.sa = sa2; r2.sb = sb2; r2.valid = valid2;
-                        s_rec[i] = r2;
+            }
+            __syncthreads();
+            if (w == f) {
+                const int n_commit = __popcll(accm & ((1ull << f) - 1ull));
+                int qA[NS], qB[NS];
+#pragma unroll
+                for (int j = 0; j < NS; ++j) { qA[j] = pA[j]; qB[j] = pB[j]; }
+                for (int c = 0; c < n_commit; ++c) {
+                    const int4 cm = s_commit[c];
+#pragma unroll
+                    for (int j = 0; j < NS; ++j) {
+                        if (sA[j] == cm.x) qA[j] = cm.z;
+                        if (sA[j] == cm.y) qA[j] = cm.w;
+                        if (sB[j] == cm.x) qB[j] = cm.z;
+                        if (sB[j] == cm.y) qB[j] = cm.w;
                     }
                 }
-                __syncthreads();
-                rec = s_rec[i];
-            }
-            bool accept = false;
-            if (rec.valid) {
-                if constexpr (sizeof(Real) == 8) accept = rec.lu < -beta * rec.dE;
-                else accept = (float)rec.lu < -(beta_f * (float)rec.dE);
-            }
-            if (accept) {
-                acc_mask |= 1 << i;
-                if (tid == 0) {
-                    const int2 pi = s_prop[i];
-                    s_commit[n_commit] = make_int4(pi.x, pi.y, rec.sb, rec.sa);
-                    occ[pi.x] = (int8_t)rec.sb;
-                    occ[pi.y] = (int8_t)rec.sa;
+                const int sa2 = __builtin_amdgcn_readfirstlane(qA[0]), sb2 = __builtin_amdgcn_readfirstlane(qB[0]);
+                const bool valid2 = sa2 != sb2;
+                double dE2 = 0.0;
+                if (valid2) dE2 = evaluate(ia, ib, sA, sB, qA, qB, sa2, sb2, near);
+                if (lane == 0) {
+                    SpecRec r2;
+                    r2.dE = dE2; r2.lu = lu; r2.sa = sa2; r2.sb = sb2; r2.valid = valid2; r2.conflict = conflict;
+                    s_rec[f] = r2;
                 }
-                ++n_commit;
-                // the commit list is read by a later re-evaluation only after that step's barrier
             }
-            if (w == 0) {
-                if (accept) {
-                    E += rec.dE;
-                    ++nacc;
+            __syncthreads();
+            if (lane == f) ri = s_rec[f];
+            const uint64_t bit = 1ull << f;
+            accm = (accm & ~bit) | (__ballot(lane == f && accepts(ri)) & bit);
+            need = __ballot(lane > f && lane < nW && (ri.conflict & (int)accm) != 0);
+        }
+        // 4. wave 0: occupation writes of the accepted steps (in parallel unless two of them share a site, which
+        // only happens through a re-evaluated step: then in step order), running total, statistics, trajectory
+        if (w == 0) {
+            const bool mine = (accm >> lane) & 1ull;
+            if (__ballot(mine && (ri.conflict & (int)accm) != 0) == 0ull) {
+                if (mine) {
+                    const int2 pl = s_prop[lane];
+                    occ[pl.x] = (int8_t)ri.sb;
+                    occ[pl.y] = (int8_t)ri.sa;
+                }
+            } else if (lane == 0) {
+                for (int i = 0; i < nW; ++i)
+                    if ((accm >> i) & 1ull) {
+                        const int2 pi = s_prop[i];
+                        const SpecRec q = s_rec[i];
+                        occ[pi.x] = (int8_t)q.sb;
+                        occ[pi.y] = (int8_t)q.sa;
+                    }
+            }
+            nacc += __popcll(accm);
+            const int lo32 = (int)(uint32_t)__double_as_longlong(ri.dE);
+            const int hi32 = (int)(__double_as_longlong(ri.dE) >> 32);
+            for (int i = 0; i < nW; ++i) {
+                if ((accm >> i) & 1ull) {
+                    const uint32_t l = (uint32_t)__builtin_amdgcn_readlane(lo32, i);
+                    const uint32_t h = (uint32_t)__builtin_amdgcn_readlane(hi32, i);
+                    E += __longlong_as_double((long long)(((uint64_t)h << 32) | l));
                 }
                 emin = fmin(emin, E);
                 if (accumulate) {
