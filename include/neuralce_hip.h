@@ -168,7 +168,8 @@ typedef struct {
     double head_b;
     int32_t readout;                   /* 0: mean over sites (+ head_b), 1: sum over sites (+ head_b) */
     double de_scale;                   /* model energy change -> total eV (ys * N for the reference models) */
-    int32_t ball_cap[8];               /* capacity (sites) of ball_l for the union of two flips, l = 0..L */
+    int32_t ball_cap[8];               /* advisory; the library derives the receptive-field sizes from the window,
+                                          the partners and the cell itself */
 } nce_nn_model_desc;
 
 /* One NeuralCE job for nce_nn_run_chunk (device pointers). */

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <unordered_set>
 #include <utility>
 #include <cstdint>
 #include <cstdio>
@@ -109,8 +110,9 @@ struct NNModelDev {
     int64_t n_cells, n_sites;
     uint32_t div12_m, div12_s, div2_m, div2_s;
     int32_t W0, J0, P, n_part, C0, L, act, readout, has_gate;
-    int32_t cin[NN_MAXL], dd[NN_MAXL];
+    int32_t cin[NN_MAXL], dd[NN_MAXL], mcat[NN_MAXL];
     int64_t off_self[NN_MAXL], off_shell[NN_MAXL], off_gate[NN_MAXL], off_b[NN_MAXL], off_gb[NN_MAXL];
+    int64_t off_wcat[NN_MAXL];   // per layer [cin][mcat]: self | shell_p / n_p | gate_p / n_p, row-major
     const int4 *win;
     const uint2 *ent_slot;       // four u16 slots, padding = 0xFFFF
     const int32_t *ent_col;
@@ -118,14 +120,19 @@ struct NNModelDev {
     const int32_t *shell_count;  // [P]
     const int32_t *part_shell;   // [n_part] shell of each partner
     const int4 *part;            // [n_part]
+    int32_t c0_eff;              // layer-0 channels with a non-zero row in the first projection
+    const int32_t *c0_ch;        // [c0_eff]
     const double *mu, *sd, *lin, *wts, *head_w;
     const float *mu_f, *sd_f, *lin_f, *wts_f, *head_w_f;
     double head_b, de_scale;
     int32_t ball_cap[NN_MAXL + 1];
-    int32_t hash_cap;            // power of two >= 2 * ball_cap[L]
-    int64_t off_h[NN_MAXL], off_e, cache_elems;
-    int64_t soff_h0, soff_src, soff_x, soff_h[NN_MAXL], soff_e, scratch_elems;
-    int32_t kmax;                // max c_in * (1 + P)
+    int32_t hash_cap;            // power of two >= 1.5 * ball_cap[L]
+    // per precision: which per-step arrays live in LDS (else in the scratch): 0 none, 1 the layer-0 change of
+    // ball 0, 2 and the sorted partner hits, 3 and the projections
+    int32_t lds_mode32, lds_mode64;
+    int64_t pr_elems, hit_elems, w_elems;   // largest projection block / hit list / used projection matrix
+    int64_t off_pre[NN_MAXL], off_gpre[NN_MAXL], cache_elems;
+    int64_t soff_dh0, soff_pr, soff_tgt, soff_list, soff_dpre[NN_MAXL], soff_dgpre[NN_MAXL], soff_dh[2], scratch_elems;
 };
 
 struct NNModelHost {
@@ -299,6 +306,12 @@ template <typename Real>
 __device__ inline Real entry_delta(const Real2<Real> *val, uint2 e) {
     const Real2<Real> a = val[e.x & 0xffff], b = val[e.x >> 16], c = val[e.y & 0xffff], d = val[e.y >> 16];
     return (a.n * b.n) * (c.n * d.n) - (a.o * b.o) * (c.o * d.o);
+}
+
+// the same for binary point functions held as int8 (-1 / +1, 0 for the excluded slots): exact in integers
+__device__ inline int entry_delta_bin(const char2 *val, uint2 e) {
+    const char2 a = val[e.x & 0xffff], b = val[e.x >> 16], c = val[e.y & 0xffff], d = val[e.y >> 16];
+    return (a.y * b.y) * (c.y * d.y) - (a.x * b.x) * (c.x * d.x);
 }
 
 // Partial sum of coef * (prod_new - prod_old) over both sites' entries for this thread.
@@ -542,7 +555,7 @@ __host__ __device__ inline SpecLds<Real> spec_lds_layout(int J, int Wcap, int wa
     size_t o = 0;
     L.slot = o;   o = align16(o + (size_t)J * sizeof(uint2));
     L.coef = o;   o = align16(o + (size_t)J * sizeof(Real));
-    L.val = o;    o = align16(o + (size_t)waves * 2 * Wcap * 2 * sizeof(Real));
+    L.val = o;    o = align16(o + (size_t)waves * 2 * Wcap * sizeof(char2));
     L.prop = o;   o = align16(o + (size_t)waves * sizeof(int2));
     L.rec = o;    o = align16(o + (size_t)waves * sizeof(SpecRec));
     L.commit = o; o = align16(o + (size_t)waves * sizeof(int4));
@@ -569,8 +582,8 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     const SpecLds<Real> L = spec_lds_layout<Real>(J, Wcap, W, nlo, nfull, ntail);
     uint2 *s_slot = (uint2 *)(smem + L.slot);
     Real *s_coef = (Real *)(smem + L.coef);
-    Real2<Real> *valA = (Real2<Real> *)(smem + L.val) + (size_t)w * 2 * Wcap;
-    Real2<Real> *valB = valA + Wcap;
+    char2 *valA = (char2 *)(smem + L.val) + (size_t)w * 2 * Wcap;   // (old, new) point-function values
+    char2 *valB = valA + Wcap;
     int2 *s_prop = (int2 *)(smem + L.prop);
     SpecRec *s_rec = (SpecRec *)(smem + L.rec);
     int4 *s_commit = (int4 *)(smem + L.commit);
@@ -589,8 +602,8 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         for (int i = tid; i < n16; i += BS) dst[i] = src[i];
     }
     if (lane == 0) {
-        valA[Wcap - 1].o = valA[Wcap - 1].n = (Real)1;
-        valB[Wcap - 1].o = valB[Wcap - 1].n = (Real)1;
+        valA[Wcap - 1] = make_char2(1, 1);
+        valB[Wcap - 1] = make_char2(1, 1);
     }
     int4 wofs[NS];
     bool wok[NS];
@@ -678,12 +691,9 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                     if (sA[j] == ib) na = sa;
                     if (sB[j] == ia) nb = sb;
                     if (sB[j] == ib) nb = sa;
-                    Real2<Real> va, vb;
-                    va.o = pA[j] ? (Real)1 : (Real)-1;
-                    va.n = na ? (Real)1 : (Real)-1;
-                    vb.o = pB[j] ? (Real)1 : (Real)-1;
-                    vb.n = nb ? (Real)1 : (Real)-1;
-                    if (sB[j] == ia) { vb.o = (Real)0; vb.n = (Real)0; }
+                    char2 va = make_char2(pA[j] ? 1 : -1, na ? 1 : -1);
+                    char2 vb = make_char2(pB[j] ? 1 : -1, nb ? 1 : -1);
+                    if (sB[j] == ia) vb = make_char2(0, 0);
                     valA[slot] = va;
                     valB[slot] = vb;
                 }
@@ -694,7 +704,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             Real acc = (Real)0;
             for (int j = lane; j < Jb0; j += 64) {
                 const uint2 e = s_slot[j];
-                acc += s_coef[j] * (entry_delta(valA, e) + entry_delta(valB, e));
+                acc += s_coef[j] * (Real)(entry_delta_bin(valA, e) + entry_delta_bin(valB, e));
             }
             sum = wave_sum(acc);
         }

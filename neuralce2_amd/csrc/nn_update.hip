@@ -1,14 +1,27 @@
 // Neural cluster expansion (NeuralCE) kernels of libneuralce_hip.so, included into neuralce_hip.hip.
 //
-//   nn_h0_kernel          standardised layer 0 of every site from the layer-0 window and reverse entries
-//   nn_layer_kernel       one hidden layer of every site (shell means of the previous layer, self + shell maps)
-//   nn_energy_kernel      last layer + site energies e_i, fixed-order partial sums
-//   nn_chunk_kernel       n_steps swap-Metropolis steps of one replica per workgroup with the receptive-field local
-//                         update: the balls of the two flipped sites are built per step in an LDS hash table (so the
-//                         two flips share one union, each site once), the layer-0 change is scattered into ball-0
-//                         rows, each layer is re-evaluated on its ball from patched inputs, and on acceptance the
-//                         ball rows are written back to the caches.
-// Layout of the per-replica cache (elements of Real): h0s [N][c0] | hidden layer l = 1..L-1 [N][d_l] | e [N].
+// Model (neuralce/nce/local_update_implicit.py:43-112, neuralce/nce/general.py:213-235): h_0 = standardised layer 0
+// (h0s); hidden layer l = 1..L at site s
+//   pre_l(s)  = h_{l-1}(s) W_self + sum_p mean_{q in shell p of s} h_{l-1}(q) W_p + b,
+//   gpre_l(s) = sum_p mean_{q in shell p of s} h_{l-1}(q) G_p + gb                  (gated models only),
+//   h_l(s)    = tanh(pre_l) * sigmoid(gpre_l)  (gate)  |  tanh(pre_l)  |  pre_l * sigmoid(pre_l)  (SiLU);
+// site energy e(s) = h0s(s) . lin + h_L(s) . w; E = sum_s e / nA + b (mean readout) or sum_s e + b.
+//
+// Per-replica cache (elements of Real): h0s [N][c0] | for each hidden layer: pre [N][d] (| gpre [N][d]).
+//
+// Kernels:
+//   nn_h0_kernel        standardised layer 0 of every site from the layer-0 window and reverse entries
+//   nn_layer_kernel     pre-activations of one hidden layer at every site
+//   nn_energy_kernel    site energies and fixed-order partial sums
+//   nn_chunk_kernel     swap-Metropolis steps of one replica per workgroup with the receptive-field local update
+//                       in incremental form.  pre_l is linear in h_{l-1}: its change on ball_l is the change of
+//                       h_{l-1} on ball_{l-1} projected through [W_self | W_p / n_p | G_p / n_p] and added at every
+//                       shell partner; the new activations follow from the cached pre-activations plus the change,
+//                       and the energy change is the layer-0 change times lin plus the last layer's change times w.
+//                       Per step: the balls of the two flips are built in an LDS hash table (two-phase claim /
+//                       assign, no spin-waits), the layer-0 change is scattered into the ball-0 rows with LDS
+//                       atomics, the partner hits of every layer are sorted by target (counting sort in LDS) so the
+//                       sums are gathers, and on acceptance the changes are added to the caches.
 
 template <typename Real>
 __device__ inline const Real *nn_w(const NNModelDev &m) {
@@ -29,7 +42,7 @@ template <typename Real>
 __device__ inline Real nn_sigmoid(Real x) { return (Real)1 / ((Real)1 + exp(-x)); }
 
 template <typename Real>
-__device__ inline Real nn_activate(const NNModelDev &m, Real pre, Real gpre) {
+__device__ inline Real nn_act(const NNModelDev &m, Real pre, Real gpre) {
     if (m.act == 1) return pre * nn_sigmoid(pre);
     Real out = tanh(pre);
     if (m.has_gate) out *= nn_sigmoid(gpre);
@@ -48,30 +61,6 @@ __device__ inline int64_t nn_site(const NNModelDev &m, int64_t center, int4 e) {
     k1 -= (k1 >= m.d1) ? m.d1 : 0;
     k2 -= (k2 >= m.d2) ? m.d2 : 0;
     return ((int64_t)k0 * m.d1 + k1) * m.d2 + k2;
-}
-
-// layer l (0-based over hidden layers) at one site and output channel d; get(site, c) returns the input row
-template <typename Real, typename Get>
-__device__ inline Real nn_layer_out(const NNModelDev &m, int l, int64_t site, int d, Get get) {
-    const Real *W = nn_w<Real>(m);
-    const int C = m.cin[l], D = m.dd[l];
-    Real pre = W[m.off_b[l] + d];
-    Real gpre = m.has_gate ? W[m.off_gb[l] + d] : (Real)0;
-    for (int c = 0; c < C; ++c) pre += get(site, c) * W[m.off_self[l] + (int64_t)c * D + d];
-    int q0 = 0;
-    for (int p = 0; p < m.P; ++p) {
-        const int np_ = m.shell_count[p];
-        const Real inv = np_ > 0 ? (Real)1 / (Real)np_ : (Real)0;
-        for (int c = 0; c < C; ++c) {
-            Real z = (Real)0;
-            for (int q = 0; q < np_; ++q) z += get(nn_site(m, site, m.part[q0 + q]), c);
-            z *= inv;
-            pre += z * W[m.off_shell[l] + ((int64_t)p * C + c) * D + d];
-            if (m.has_gate) gpre += z * W[m.off_gate[l] + ((int64_t)p * C + c) * D + d];
-        }
-        q0 += np_;
-    }
-    return nn_activate<Real>(m, pre, gpre);
 }
 
 // ── full cache ─────────────────────────────────────────────────────────────────────────────────────────────
@@ -107,32 +96,55 @@ __global__ __launch_bounds__(NN_H0_SITES) void nn_h0_kernel(const NNModelDev *__
     }
 }
 
+// pre-activations of hidden layer l at every site (inputs: h0s for l = 0, else the activations of layer l - 1)
 template <typename Real>
 __global__ __launch_bounds__(256) void nn_layer_kernel(const NNModelDev *__restrict__ models, int model, int l,
                                                        int64_t N, Real *__restrict__ cache_all) {
     const NNModelDev &m = models[model];
     const int r = blockIdx.y;
-    const int D = m.dd[l];
+    const int D = m.dd[l], C = m.cin[l];
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (idx >= N * D) return;
     const int64_t site = idx / D;
     const int d = (int)(idx - site * D);
     Real *cache = cache_all + (int64_t)r * m.cache_elems;
-    const Real *in = cache + (l == 0 ? 0 : m.off_h[l]);
-    const int C = m.cin[l];
-    auto get = [&](int64_t x, int c) -> Real { return in[x * C + c]; };
-    cache[m.off_h[l + 1] + site * D + d] = nn_layer_out<Real>(m, l, site, d, get);
+    const Real *W = nn_w<Real>(m);
+    const Real *ip = cache + (l == 0 ? 0 : m.off_pre[l - 1]);
+    const Real *ig = (l > 0 && m.has_gate) ? cache + m.off_gpre[l - 1] : nullptr;
+    auto get = [&](int64_t x, int c) -> Real {
+        if (l == 0) return ip[x * C + c];
+        return nn_act<Real>(m, ip[x * C + c], ig ? ig[x * C + c] : (Real)0);
+    };
+    Real pre = W[m.off_b[l] + d];
+    Real gpre = m.has_gate ? W[m.off_gb[l] + d] : (Real)0;
+    for (int c = 0; c < C; ++c) pre += get(site, c) * W[m.off_self[l] + (int64_t)c * D + d];
+    int q0 = 0;
+    for (int p = 0; p < m.P; ++p) {
+        const int np_ = m.shell_count[p];
+        const Real inv = np_ > 0 ? (Real)1 / (Real)np_ : (Real)0;
+        for (int c = 0; c < C; ++c) {
+            Real z = (Real)0;
+            for (int q = 0; q < np_; ++q) z += get(nn_site(m, site, m.part[q0 + q]), c);
+            z *= inv;
+            pre += z * W[m.off_shell[l] + ((int64_t)p * C + c) * D + d];
+            if (m.has_gate) gpre += z * W[m.off_gate[l] + ((int64_t)p * C + c) * D + d];
+        }
+        q0 += np_;
+    }
+    cache[m.off_pre[l] + site * D + d] = pre;
+    if (m.has_gate) cache[m.off_gpre[l] + site * D + d] = gpre;
 }
 
-// site energies (last layer evaluated here) and per-block partial sums in fixed order
+// site energies and per-block partial sums in fixed order
 template <typename Real>
 __global__ __launch_bounds__(256) void nn_energy_kernel(const NNModelDev *__restrict__ models, int model, int64_t N,
-                                                        Real *__restrict__ cache_all, double *__restrict__ partial) {
+                                                        const Real *__restrict__ cache_all,
+                                                        double *__restrict__ partial) {
     __shared__ double red[4];
     const NNModelDev &m = models[model];
     const int r = blockIdx.y;
     const int64_t site = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    Real *cache = cache_all + (int64_t)r * m.cache_elems;
+    const Real *cache = cache_all + (int64_t)r * m.cache_elems;
     double e = 0.0;
     if (site < N) {
         const Real *h0 = cache + site * m.C0;
@@ -140,13 +152,12 @@ __global__ __launch_bounds__(256) void nn_energy_kernel(const NNModelDev *__rest
         Real acc = (Real)0;
         for (int c = 0; c < m.C0; ++c) acc += h0[c] * lin[c];
         if (m.L > 0) {
-            const int l = m.L - 1, C = m.cin[l];
-            const Real *in = cache + (l == 0 ? 0 : m.off_h[l]);
-            auto get = [&](int64_t x, int c) -> Real { return in[x * C + c]; };
+            const int l = m.L - 1, D = m.dd[l];
+            const Real *pre = cache + m.off_pre[l] + site * D;
+            const Real *gp = m.has_gate ? cache + m.off_gpre[l] + site * D : nullptr;
             const Real *hw = nn_headw<Real>(m);
-            for (int d = 0; d < m.dd[l]; ++d) acc += nn_layer_out<Real>(m, l, site, d, get) * hw[d];
+            for (int d = 0; d < D; ++d) acc += nn_act<Real>(m, pre[d], gp ? gp[d] : (Real)0) * hw[d];
         }
-        cache[m.off_e + site] = acc;
         e = (double)acc;
     }
     e = wave_sum(e);
@@ -168,21 +179,34 @@ __global__ void nn_energy_finish_kernel(const NNModelDev *__restrict__ models, i
 // ── local update chunk kernel ─────────────────────────────────────────────────────────────────────────────
 constexpr int NN_BS = 512;
 constexpr int NN_EMPTY = -1;
+constexpr int NN_RB = 4;                 // rows per thread in the projection
 
 struct NNLds {
-    size_t keys, vals, slot_site, slot_pos, red, cnt, total;
+    size_t keys, vals, slot_site, cnt, red, misc, w, dh0, list, pr, total;
 };
-__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap) {
+__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, size_t w_bytes, size_t dh0_bytes,
+                                               size_t list_bytes, size_t pr_bytes) {
     NNLds L;
     size_t o = 0;
     L.keys = o;      o = align16(o + (size_t)hash_cap * sizeof(int));
     L.vals = o;      o = align16(o + (size_t)hash_cap * sizeof(int));
     L.slot_site = o; o = align16(o + (size_t)ball_cap * sizeof(int));
-    L.slot_pos = o;  o = align16(o + (size_t)ball_cap * sizeof(int));
-    L.red = o;       o = align16(o + (NN_BS / 64) * sizeof(double));
-    L.cnt = o;       o = align16(o + 4 * sizeof(int));
+    L.cnt = o;       o = align16(o + (size_t)(ball_cap + 1) * sizeof(int));
+    L.red = o;       o = align16(o + (NN_BS / 64 + 1) * sizeof(double));
+    L.misc = o;      o = align16(o + 4 * sizeof(int));
+    L.w = o;         o = align16(o + w_bytes);
+    L.dh0 = o;       o = align16(o + dh0_bytes);
+    L.list = o;      o = align16(o + list_bytes);
+    L.pr = o;        o = align16(o + pr_bytes);
     L.total = o;
     return L;
+}
+template <typename Real>
+__host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode) {
+    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], (size_t)m.w_elems * sizeof(Real),
+                         mode >= 1 ? (size_t)m.ball_cap[0] * m.C0 * sizeof(Real) : 0,
+                         mode >= 2 ? (size_t)m.hit_elems * sizeof(int) : 0,
+                         mode >= 3 ? (size_t)m.pr_elems * sizeof(Real) : 0);
 }
 
 __device__ inline uint32_t nn_hash(int site, int mask) { return ((uint32_t)site * 2654435761u) & (uint32_t)mask; }
@@ -220,7 +244,37 @@ __device__ inline Real nn_block_sum(Real v, Real *red) {
     return s;
 }
 
-template <typename Real>
+// in-place exclusive prefix sum of a[0..n) (block-wide); returns the total
+__device__ inline int nn_block_scan(int *a, int n, double *red_d) {
+    int *red = (int *)red_d;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int seg = (n + NN_BS - 1) / NN_BS;
+    const int b = tid * seg, e = min(b + seg, n);
+    int loc = 0;
+    for (int i = b; i < e; ++i) loc += a[i];
+    int incl = loc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) red[w] = incl;
+    __syncthreads();
+    int wave_off = 0, total = 0;
+    for (int i = 0; i < NN_BS / 64; ++i) {
+        if (i < w) wave_off += red[i];
+        total += red[i];
+    }
+    int run = wave_off + incl - loc;
+    for (int i = b; i < e; ++i) {
+        const int v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+template <typename Real, int MODE>
 __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__restrict__ models,
                                                          const nce_nn_job *__restrict__ jobs,
                                                          const int2 *__restrict__ items, int64_t n_steps) {
@@ -230,26 +284,27 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     const NNModelDev &m = models[job.model];
     const int r = it.y;
     const int tid = threadIdx.x;
-    const int L = m.L, C0 = m.C0;
+    const int L = m.L, C0 = m.C0, n_part = m.n_part, P = m.P;
     const int mask = m.hash_cap - 1;
-    const NNLds lay = nn_lds_layout(m.hash_cap, m.ball_cap[L]);
+    const NNLds lay = nn_lds_for<Real>(m, MODE);
     int *keys = (int *)(smem + lay.keys);
     int *vals = (int *)(smem + lay.vals);
     int *slot_site = (int *)(smem + lay.slot_site);
-    int *slot_pos = (int *)(smem + lay.slot_pos);
-    double *red = (double *)(smem + lay.red);
     int *cnt = (int *)(smem + lay.cnt);
+    double *red = (double *)(smem + lay.red);
+    int *misc = (int *)(smem + lay.misc);
     for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
-    if (tid == 0) cnt[0] = 0;
+    if (tid == 0) misc[0] = 0;
     __syncthreads();
 
     int8_t *occ = job.occ + (int64_t)r * job.n_sites;
     Real *cache = (Real *)job.cache + (int64_t)r * m.cache_elems;
     Real *scr = (Real *)job.scratch + (int64_t)r * m.scratch_elems;
-    Real *h0new = scr + m.soff_h0;
-    int *src = (int *)(scr + m.soff_src);
-    Real *X = scr + m.soff_x;
-    Real *enew = scr + m.soff_e;
+    Real *wl = (Real *)(smem + lay.w);                 // projection matrix of the current layer
+    Real *dh0 = MODE >= 1 ? (Real *)(smem + lay.dh0) : scr + m.soff_dh0;
+    int *list = MODE >= 2 ? (int *)(smem + lay.list) : (int *)(scr + m.soff_list);
+    Real *pr = MODE >= 3 ? (Real *)(smem + lay.pr) : scr + m.soff_pr;
+    int *tgt = (int *)(scr + m.soff_tgt);
     const Real *W = nn_w<Real>(m);
     const Real *lin = nn_lin<Real>(m);
     const Real *hw = nn_headw<Real>(m);
@@ -259,22 +314,6 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int64_t nacc = 0;
     const int64_t inj_base = (int64_t)r * n_steps;
     int counts[NN_MAXL + 1];
-
-    // insert `n` candidate sites given by site_fn(k); on return the new slots are numbered and cnt[0] updated
-    auto insert_sites = [&](int n, auto site_fn) {
-        for (int k = tid; k < n; k += NN_BS) {
-            const int site = site_fn(k);
-            bool claimed;
-            const int pos = nn_hash_claim(keys, mask, site, &claimed);
-            if (claimed) {
-                const int s = atomicAdd(&cnt[0], 1);
-                vals[pos] = s;
-                slot_site[s] = site;
-                slot_pos[s] = pos;
-            }
-        }
-        __syncthreads();
-    };
 
     for (int64_t t = 0; t < n_steps; ++t) {
         int64_t ia, ib;
@@ -290,26 +329,31 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             ++nacc;
         } else {
             // ball 0: the two layer-0 windows
-            insert_sites(2 * m.W0, [&](int k) {
-                return (int)nn_site(m, k < m.W0 ? ia : ib, m.win[k < m.W0 ? k : k - m.W0]);
-            });
-            counts[0] = cnt[0];
-            // h0new = cached standardised rows of ball 0, then the layer-0 change of flip 1 and of flip 2
-            for (int k = tid; k < counts[0] * C0; k += NN_BS) {
-                const int s = k / C0, c = k - s * C0;
-                h0new[k] = cache[(int64_t)slot_site[s] * C0 + c];
+            for (int k = tid; k < 2 * m.W0; k += NN_BS) {
+                const int site = (int)nn_site(m, k < m.W0 ? ia : ib, m.win[k < m.W0 ? k : k - m.W0]);
+                bool claimed;
+                const int pos = nn_hash_claim(keys, mask, site, &claimed);
+                if (claimed) {
+                    const int s = atomicAdd(&misc[0], 1);
+                    vals[pos] = s;
+                    slot_site[s] = site;
+                }
             }
             __syncthreads();
+            counts[0] = misc[0];
+            for (int k = tid; k < counts[0] * C0; k += NN_BS) dh0[k] = (Real)0;
+            __syncthreads();
+            // layer-0 change of flip 1, then of flip 2 with flip 1 applied
             for (int f = 0; f < 2; ++f) {
                 const int64_t fs = f == 0 ? ia : ib;
                 for (int e = tid; e < m.J0; e += NN_BS) {
                     const uint2 sl = m.ent_slot[e];
                     const uint32_t s4[4] = {sl.x & 0xffffu, sl.x >> 16, sl.y & 0xffffu, sl.y >> 16};
-                    int64_t xs[4];
+                    int xs[4];
                     double po = 1.0, pn = 1.0;
                     for (int q = 0; q < 4; ++q) {
                         if (s4[q] == 0xffffu) { xs[q] = -1; continue; }
-                        const int64_t x = nn_site(m, fs, m.win[s4[q]]);
+                        const int x = (int)nn_site(m, fs, m.win[s4[q]]);
                         xs[q] = x;
                         int sp = occ[x];
                         if (f == 1 && x == ia) sp = sj;           // flip 2 sees flip 1 applied
@@ -324,83 +368,140 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         const Real v = (Real)(dv / m.sd[col]);
                         for (int q = 0; q < 4; ++q) {
                             if (xs[q] < 0) continue;
-                            const int s = nn_hash_find(keys, vals, mask, (int)xs[q]);
-                            atomicAdd(&h0new[(int64_t)s * C0 + col], v);
+                            const int s = nn_hash_find(keys, vals, mask, xs[q]);
+                            atomicAdd(&dh0[(int64_t)s * C0 + col], v);
                         }
                     }
                 }
             }
             __syncthreads();
-            // hidden layers on the growing balls
-            for (int l = 0; l < L; ++l) {
-                const int Cin = m.cin[l], D = m.dd[l];
-                const int prev = counts[l];
-                insert_sites(prev * m.n_part, [&](int k) {
-                    const int s = k / m.n_part, q = k - s * m.n_part;
-                    return (int)nn_site(m, slot_site[s], m.part[q]);
-                });
-                counts[l + 1] = cnt[0];
-                const int nb = counts[l + 1];
-                const Real *pnew = l == 0 ? h0new : scr + m.soff_h[l - 1];
-                const Real *pcache = cache + (l == 0 ? 0 : m.off_h[l]);
-                // source of every (ball site, partner) input: slot of the previous ball or cache row (-1 - site)
-                for (int k = tid; k < nb * (1 + m.n_part); k += NN_BS) {
-                    const int s = k / (1 + m.n_part), q = k - s * (1 + m.n_part);
-                    const int x = q == 0 ? slot_site[s] : (int)nn_site(m, slot_site[s], m.part[q - 1]);
-                    const int sx = nn_hash_find(keys, vals, mask, x);
-                    src[k] = (sx >= 0 && sx < prev) ? sx : -1 - x;
-                }
-                __syncthreads();
-                // inputs X[s] = [h(s) | mean over shell p of h(partners)] for every ball site
-                const int K = Cin * (1 + m.P);
-                for (int k = tid; k < nb * Cin; k += NN_BS) {
-                    const int s = k / Cin, c = k - s * Cin;
-                    const int *sr = src + (int64_t)s * (1 + m.n_part);
-                    auto val = [&](int code) -> Real {
-                        return code >= 0 ? pnew[(int64_t)code * Cin + c] : pcache[(int64_t)(-1 - code) * Cin + c];
-                    };
-                    X[(int64_t)s * K + c] = val(sr[0]);
-                    int q0 = 0;
-                    for (int p = 0; p < m.P; ++p) {
-                        const int np_ = m.shell_count[p];
-                        Real z = (Real)0;
-                        for (int q = 0; q < np_; ++q) z += val(sr[1 + q0 + q]);
-                        X[(int64_t)s * K + Cin * (1 + p) + c] = np_ > 0 ? z / (Real)np_ : (Real)0;
-                        q0 += np_;
-                    }
-                }
-                __syncthreads();
-                Real *hnew = scr + m.soff_h[l];
-                for (int k = tid; k < nb * D; k += NN_BS) {
-                    const int s = k / D, d = k - s * D;
-                    const Real *x = X + (int64_t)s * K;
-                    Real pre = W[m.off_b[l] + d];
-                    Real gpre = m.has_gate ? W[m.off_gb[l] + d] : (Real)0;
-                    for (int c = 0; c < Cin; ++c) pre += x[c] * W[m.off_self[l] + (int64_t)c * D + d];
-                    for (int p = 0; p < m.P; ++p) {
-                        const Real *z = x + Cin * (1 + p);
-                        for (int c = 0; c < Cin; ++c) {
-                            pre += z[c] * W[m.off_shell[l] + ((int64_t)p * Cin + c) * D + d];
-                            if (m.has_gate) gpre += z[c] * W[m.off_gate[l] + ((int64_t)p * Cin + c) * D + d];
-                        }
-                    }
-                    hnew[k] = nn_activate<Real>(m, pre, gpre);
-                }
-                __syncthreads();
-            }
-            // site energies on the last ball, energy change
-            const int nL = counts[L];
-            const Real *hL = L > 0 ? scr + m.soff_h[L - 1] : nullptr;
-            const int DL = L > 0 ? m.dd[L - 1] : 0;
             double part = 0.0;
-            for (int s = tid; s < nL; s += NN_BS) {
-                const int x = slot_site[s];
-                const Real *h0 = s < counts[0] ? h0new + (int64_t)s * C0 : cache + (int64_t)x * C0;
-                Real e = (Real)0;
-                for (int c = 0; c < C0; ++c) e += h0[c] * lin[c];
-                for (int d = 0; d < DL; ++d) e += hL[(int64_t)s * DL + d] * hw[d];
-                enew[s] = e;
-                part += (double)e - (double)cache[m.off_e + x];
+            for (int k = tid; k < counts[0] * C0; k += NN_BS) part += (double)(dh0[k] * lin[k % C0]);
+            // hidden layers: projection of the previous layer's change, partner hits sorted by target, gather
+            for (int l = 0; l < L; ++l) {
+                const int Cin = l == 0 ? m.c0_eff : m.cin[l], D = m.dd[l], M = m.mcat[l];
+                const int prev = counts[l];
+                const Real *dx = l == 0 ? dh0 : scr + m.soff_dh[(l - 1) & 1];
+                const int xs = l == 0 ? C0 : m.cin[l];
+                const Real *Wc = W + m.off_wcat[l];
+                // (1) pr[q][j] = dx[q] . Wcat[:, j]: the layer's matrix (used rows only) staged in LDS, 4 x 4
+                // register tiles
+                for (int k = tid; k < Cin * M; k += NN_BS) {
+                    const int c = k / M, j = k - c * M;
+                    wl[k] = Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j];
+                }
+                __syncthreads();
+                const int nqb = (prev + NN_RB - 1) / NN_RB, ncb = (M + 3) / 4;
+                for (int k = tid; k < nqb * ncb; k += NN_BS) {
+                    const int qb = k / ncb, j0 = (k - qb * ncb) * 4;
+                    const int q0 = qb * NN_RB;
+                    Real acc[NN_RB][4];
+#pragma unroll
+                    for (int i = 0; i < NN_RB; ++i)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) acc[i][u] = (Real)0;
+                    for (int c = 0; c < Cin; ++c) {
+                        const int cx = l == 0 ? m.c0_ch[c] : c;
+                        Real wv[4], xv[NN_RB];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) wv[u] = j0 + u < M ? wl[c * M + j0 + u] : (Real)0;
+#pragma unroll
+                        for (int i = 0; i < NN_RB; ++i) xv[i] = q0 + i < prev ? dx[(int64_t)(q0 + i) * xs + cx] : (Real)0;
+#pragma unroll
+                        for (int i = 0; i < NN_RB; ++i)
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) acc[i][u] += xv[i] * wv[u];
+                    }
+#pragma unroll
+                    for (int i = 0; i < NN_RB; ++i)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (q0 + i < prev && j0 + u < M) pr[(int64_t)(q0 + i) * M + j0 + u] = acc[i][u];
+                }
+                // (2) partner sites of ball_l: claim (new sites get the next slots), then slot of every hit
+                const int nh = prev * n_part;
+                for (int i = tid; i < nh; i += NN_BS) {
+                    const int q = i / n_part, kk = i - q * n_part;
+                    const int site = (int)nn_site(m, slot_site[q], m.part[kk]);
+                    bool claimed;
+                    const int pos = nn_hash_claim(keys, mask, site, &claimed);
+                    if (claimed) {
+                        const int s = atomicAdd(&misc[0], 1);
+                        vals[pos] = s;
+                        slot_site[s] = site;
+                    }
+                    tgt[i] = pos;
+                }
+                __syncthreads();
+                const int nb = misc[0];
+                counts[l + 1] = nb;
+                for (int s = tid; s < nb; s += NN_BS) cnt[s] = 0;
+                __syncthreads();
+                for (int i = tid; i < nh; i += NN_BS) {
+                    const int s = vals[tgt[i]];
+                    tgt[i] = s;
+                    atomicAdd(&cnt[s], 1);
+                }
+                __syncthreads();
+                // (3) counting sort of the hits by target: after the fill cnt[s] is the end of target s's range
+                nn_block_scan(cnt, nb, red);
+                for (int i = tid; i < nh; i += NN_BS) {
+                    const int q = i / n_part, kk = i - q * n_part;
+                    const int pos = atomicAdd(&cnt[tgt[i]], 1);
+                    list[pos] = (q << 8) | m.part_shell[kk];
+                }
+                __syncthreads();
+                // (4) change of the pre-activations, new activations, change of the layer output
+                Real *dpre = scr + m.soff_dpre[l];
+                Real *dgpre = scr + m.soff_dgpre[l];
+                Real *dh = scr + m.soff_dh[l & 1];
+                const Real *cpre = cache + m.off_pre[l];
+                const Real *cgpre = cache + m.off_gpre[l];
+                const bool last = l + 1 == L;
+                const int og = D * (1 + P);
+                constexpr int U = 4;                     // items per thread in flight together
+                for (int k0 = tid; k0 < nb * D; k0 += U * NN_BS) {
+                    Real p0[U], g0[U];
+                    int sv[U], dv[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + u * NN_BS;
+                        const int kk = k < nb * D ? k : 0;
+                        sv[u] = kk / D; dv[u] = kk - sv[u] * D;
+                        const int64_t o = (int64_t)slot_site[sv[u]] * D + dv[u];
+                        p0[u] = cpre[o];
+                        g0[u] = m.has_gate ? cgpre[o] : (Real)0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int k = k0 + u * NN_BS;
+                        if (k >= nb * D) break;
+                        const int s = sv[u], d = dv[u];
+                        Real dp = s < prev ? pr[(int64_t)s * M + d] : (Real)0;
+                        Real dg = (Real)0;
+                        const int j0 = s == 0 ? 0 : cnt[s - 1], j1 = cnt[s];
+                        // hits four at a time: independent loads in flight together
+                        auto at = [&](int h) -> int64_t { return (int64_t)(h >> 8) * M + D * (h & 0xff) + d; };
+                        int j = j0;
+                        for (; j + 4 <= j1; j += 4) {
+                            const int64_t a0 = at(list[j]), a1 = at(list[j + 1]), a2 = at(list[j + 2]),
+                                          a3 = at(list[j + 3]);
+                            dp += (pr[a0 + D] + pr[a1 + D]) + (pr[a2 + D] + pr[a3 + D]);
+                            if (m.has_gate) dg += (pr[a0 + og] + pr[a1 + og]) + (pr[a2 + og] + pr[a3 + og]);
+                        }
+                        for (; j < j1; ++j) {
+                            const int64_t a = at(list[j]);
+                            dp += pr[a + D];
+                            if (m.has_gate) dg += pr[a + og];
+                        }
+                        const Real dhv = nn_act<Real>(m, p0[u] + dp, g0[u] + dg) - nn_act<Real>(m, p0[u], g0[u]);
+                        dpre[k] = dp;
+                        if (m.has_gate) dgpre[k] = dg;
+                        if (last) part += (double)(dhv * hw[d]);
+                        else dh[k] = dhv;
+                    }
+                }
+                __syncthreads();
             }
             const double diff = nn_block_sum<double>(part, red);
             const double dE = (m.readout == 0 ? diff / (double)m.nA : diff);
@@ -409,17 +510,19 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             if (accept) {
                 for (int k = tid; k < counts[0] * C0; k += NN_BS) {
                     const int s = k / C0, c = k - s * C0;
-                    cache[(int64_t)slot_site[s] * C0 + c] = h0new[k];
+                    cache[(int64_t)slot_site[s] * C0 + c] += dh0[k];
                 }
-                for (int l = 0; l + 1 < L; ++l) {
+                for (int l = 0; l < L; ++l) {
                     const int D = m.dd[l];
-                    const Real *hnew = scr + m.soff_h[l];
+                    const Real *dpre = scr + m.soff_dpre[l];
+                    const Real *dgpre = scr + m.soff_dgpre[l];
                     for (int k = tid; k < counts[l + 1] * D; k += NN_BS) {
                         const int s = k / D, d = k - s * D;
-                        cache[m.off_h[l + 1] + (int64_t)slot_site[s] * D + d] = hnew[k];
+                        const int64_t o = (int64_t)slot_site[s] * D + d;
+                        cache[m.off_pre[l] + o] += dpre[k];
+                        if (m.has_gate) cache[m.off_gpre[l] + o] += dgpre[k];
                     }
                 }
-                for (int s = tid; s < nL; s += NN_BS) cache[m.off_e + slot_site[s]] = enew[s];
                 if (tid == 0) {
                     occ[ia] = (int8_t)sj;
                     occ[ib] = (int8_t)si;
@@ -428,9 +531,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 ++nacc;
             }
             // clear the table for the next step
-            for (int s = tid; s < nL; s += NN_BS) keys[slot_pos[s]] = NN_EMPTY;
-            __syncthreads();
-            if (tid == 0) cnt[0] = 0;
+            for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
+            if (tid == 0) misc[0] = 0;
         }
         if (job.traj && ((t + 1) % job.sample_interval) == 0 && tid == 0)
             job.traj[((t + 1) / job.sample_interval - 1) * job.n_replicas + r] = E;
@@ -458,11 +560,21 @@ static int nn_sync_models(nce_ctx *c) {
     return NCE_OK;
 }
 
+// LDS placement of the chunk kernel's per-step arrays: the most that fits beside the hash table, in the order
+// layer-0 change, partner hits, projections
+template <typename Real>
+static int nn_lds_mode(const NNModelDev &m) {
+    int mode = 3;
+    while (mode > 0 && nn_lds_for<Real>(m, mode).total > 160 * 1024) --mode;
+    return mode;
+}
+
 extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32_t *model_id) {
     if (!c || !d || !model_id) return set_err(NCE_ERR_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(c->device));
     if (d->n_layers < 0 || d->n_layers > NN_MAXL - 1) return set_err(NCE_ERR_ARG, "0..7 hidden layers supported");
-    if (d->n_window < 1 || d->n_window >= 0xFFFF || d->n_entries < 1 || d->c0 < 1 || d->n_shells < 0)
+    if (d->n_window < 1 || d->n_window >= 0xFFFF || d->n_entries < 1 || d->c0 < 1 || d->n_shells < 0 ||
+        d->n_shells > 255)
         return set_err(NCE_ERR_ARG, "model sizes out of range");
     for (int i = 0; i < 3; ++i)
         if (d->d[i] < 1) return set_err(NCE_ERR_ARG, "cell diagonal must be positive");
@@ -482,29 +594,56 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
         n_part += d->shell_count[p];
     }
     m.n_part = n_part;
-    // weights: per layer self [cin][d], shell [P][cin][d], gate [P][cin][d], b [d], gb [d]
+    const int64_t N = m.n_sites;
+    // weights: per layer self [cin][d], shell [P][cin][d], gate [P][cin][d], b [d], gb [d], then the projection
+    // matrix [cin][mcat] = self | shell_p / n_p | gate_p / n_p
     std::vector<double> wts;
     int64_t so = 0, sho = 0, go = 0, bo = 0, gbo = 0;
-    int cin_prev = m.C0, kmax = 0;
+    int cin_prev = m.C0;
+    std::vector<int32_t> c0_ch;
     for (int l = 0; l < m.L; ++l) {
         const int C = d->layer_cin[l], D = d->layer_d[l];
         if (C != cin_prev || D < 1) return set_err(NCE_ERR_ARG, "layer widths do not chain");
         m.cin[l] = C; m.dd[l] = D;
-        kmax = std::max(kmax, C * (1 + m.P));
+        const int M = D * (1 + m.P + (m.has_gate ? m.P : 0));
+        m.mcat[l] = M;
+        const double *ws = d->w_self + so, *wsh = d->w_shell + sho, *wg = m.has_gate ? d->w_gate + go : nullptr;
         m.off_self[l] = (int64_t)wts.size();
-        wts.insert(wts.end(), d->w_self + so, d->w_self + so + (int64_t)C * D); so += (int64_t)C * D;
+        wts.insert(wts.end(), ws, ws + (int64_t)C * D);
         m.off_shell[l] = (int64_t)wts.size();
-        wts.insert(wts.end(), d->w_shell + sho, d->w_shell + sho + (int64_t)m.P * C * D); sho += (int64_t)m.P * C * D;
+        wts.insert(wts.end(), wsh, wsh + (int64_t)m.P * C * D);
         m.off_gate[l] = (int64_t)wts.size();
-        if (m.has_gate) { wts.insert(wts.end(), d->w_gate + go, d->w_gate + go + (int64_t)m.P * C * D); go += (int64_t)m.P * C * D; }
+        if (m.has_gate) wts.insert(wts.end(), wg, wg + (int64_t)m.P * C * D);
         m.off_b[l] = (int64_t)wts.size();
-        wts.insert(wts.end(), d->b + bo, d->b + bo + D); bo += D;
+        wts.insert(wts.end(), d->b + bo, d->b + bo + D);
         m.off_gb[l] = (int64_t)wts.size();
-        if (m.has_gate) { wts.insert(wts.end(), d->gb + gbo, d->gb + gbo + D); gbo += D; }
+        if (m.has_gate) wts.insert(wts.end(), d->gb + gbo, d->gb + gbo + D);
+        m.off_wcat[l] = (int64_t)wts.size();
+        std::vector<double> wc((size_t)C * M, 0.0);
+        for (int cc = 0; cc < C; ++cc) {
+            for (int dd = 0; dd < D; ++dd) wc[(size_t)cc * M + dd] = ws[(int64_t)cc * D + dd];
+            for (int p = 0; p < m.P; ++p) {
+                const double inv = d->shell_count[p] > 0 ? 1.0 / d->shell_count[p] : 0.0;
+                for (int dd = 0; dd < D; ++dd) {
+                    wc[(size_t)cc * M + D * (1 + p) + dd] = wsh[((int64_t)p * C + cc) * D + dd] * inv;
+                    if (m.has_gate) wc[(size_t)cc * M + D * (1 + m.P + p) + dd] = wg[((int64_t)p * C + cc) * D + dd] * inv;
+                }
+            }
+        }
+        if (l == 0)
+            for (int cc = 0; cc < C; ++cc) {
+                bool nz = false;
+                for (int j = 0; j < M; ++j) nz |= wc[(size_t)cc * M + j] != 0.0;
+                if (nz) c0_ch.push_back(cc);
+            }
+        wts.insert(wts.end(), wc.begin(), wc.end());
+        so += (int64_t)C * D; sho += (int64_t)m.P * C * D; bo += D;
+        if (m.has_gate) { go += (int64_t)m.P * C * D; gbo += D; }
         cin_prev = D;
     }
     if (wts.empty()) wts.push_back(0.0);
-    m.kmax = kmax;
+    if (c0_ch.empty()) c0_ch.push_back(0);
+    m.c0_eff = m.L > 0 ? (int32_t)c0_ch.size() : 0;
     const int DL = m.L > 0 ? m.dd[m.L - 1] : 0;
     // slots
     std::vector<uint2> slots(m.J0);
@@ -528,28 +667,60 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     std::vector<int32_t> part_shell(std::max(n_part, 1));
     for (int p = 0, k = 0; p < m.P; ++p)
         for (int q = 0; q < d->shell_count[p]; ++q) part_shell[k++] = p;
-    for (int l = 0; l <= m.L; ++l) {
-        if (d->ball_cap[l] < 1) return set_err(NCE_ERR_ARG, "ball capacity");
-        m.ball_cap[l] = d->ball_cap[l];
+    // balls: the sites of one flip's ball_l on this cell (window, then dilated by the partner offsets l times, as
+    // the kernel builds it); the union of two flips holds at most twice as many, and never more than the cell
+    {
+        auto site_of = [&](int64_t centre, const int32_t *o) {
+            const int64_t k2c = centre % m.d2, k1c = (centre / m.d2) % m.d1, k0c = centre / ((int64_t)m.d1 * m.d2);
+            const int64_t k0 = (k0c + o[1]) % m.d0, k1 = (k1c + o[2]) % m.d1, k2 = (k2c + o[3]) % m.d2;
+            return (k0 * m.d1 + k1) * m.d2 + k2;
+        };
+        std::unordered_set<int64_t> ball;
+        std::vector<int64_t> cur;
+        for (int k = 0; k < m.W0; ++k)
+            if (ball.insert(site_of(0, d->window + 4 * k)).second) cur.push_back(site_of(0, d->window + 4 * k));
+        for (int l = 0; l <= m.L; ++l) {
+            if (l > 0) {
+                const std::vector<int64_t> prev(cur);
+                for (int64_t x : prev)
+                    for (int k = 0; k < n_part; ++k) {
+                        const int64_t y = site_of(x, d->partners + 4 * k);
+                        if (ball.insert(y).second) cur.push_back(y);
+                    }
+            }
+            m.ball_cap[l] = (int32_t)std::min<int64_t>(2 * (int64_t)cur.size(), N);
+            if ((int64_t)m.ball_cap[l] >= ((int64_t)1 << 23))
+                return set_err(NCE_ERR_UNSUPPORTED, "receptive field too large");
+        }
     }
-    int hc = 1;
-    while (hc < 2 * m.ball_cap[m.L]) hc <<= 1;
+    int hc = 64;
+    while (hc < m.ball_cap[m.L] + m.ball_cap[m.L] / 2) hc <<= 1;
     m.hash_cap = hc;
     // per-replica cache layout
-    const int64_t N = m.n_sites;
     int64_t o = N * m.C0;
-    for (int l = 0; l + 1 < m.L; ++l) { m.off_h[l + 1] = o; o += N * m.dd[l]; }
-    m.off_e = o; o += N;
+    for (int l = 0; l < m.L; ++l) {
+        m.off_pre[l] = o; o += N * m.dd[l];
+        m.off_gpre[l] = o; if (m.has_gate) o += N * m.dd[l];
+    }
     m.cache_elems = o;
-    // per-replica scratch layout
-    int64_t so2 = 0;
-    m.soff_h0 = so2; so2 += (int64_t)m.ball_cap[0] * m.C0;
-    m.soff_src = so2; so2 += (int64_t)m.ball_cap[m.L] * (1 + n_part);     // one int per Real-sized element
-    m.soff_x = so2; so2 += (int64_t)m.ball_cap[m.L] * std::max(kmax, 1);
-    for (int l = 0; l < m.L; ++l) { m.soff_h[l] = so2; so2 += (int64_t)m.ball_cap[l + 1] * m.dd[l]; }
-    m.soff_e = so2; so2 += m.ball_cap[m.L];
-    m.scratch_elems = so2;
-    (void)DL;
+    // per-replica scratch layout (elements of the precision's type; int arrays use one element per int)
+    int64_t s2 = 0, pr_max = 1, hit_max = 1, dh_max = 1;
+    for (int l = 0; l < m.L; ++l) {
+        pr_max = std::max(pr_max, (int64_t)m.ball_cap[l] * m.mcat[l]);
+        hit_max = std::max(hit_max, (int64_t)m.ball_cap[l] * std::max(n_part, 1));
+        dh_max = std::max(dh_max, (int64_t)m.ball_cap[l + 1] * m.dd[l]);
+    }
+    m.soff_dh0 = s2; s2 += (int64_t)m.ball_cap[0] * m.C0;
+    m.soff_pr = s2; s2 += pr_max;
+    m.soff_tgt = s2; s2 += hit_max;
+    m.soff_list = s2; s2 += hit_max;
+    for (int l = 0; l < m.L; ++l) {
+        m.soff_dpre[l] = s2; s2 += (int64_t)m.ball_cap[l + 1] * m.dd[l];
+        m.soff_dgpre[l] = s2; if (m.has_gate) s2 += (int64_t)m.ball_cap[l + 1] * m.dd[l];
+    }
+    m.soff_dh[0] = s2; s2 += dh_max;
+    m.soff_dh[1] = s2; s2 += dh_max;
+    m.scratch_elems = s2;
     std::vector<float> mu_f(m.C0), sd_f(m.C0), lin_f(m.C0), w_f(wts.size()), hw_f(std::max(DL, 1));
     for (int i = 0; i < m.C0; ++i) { mu_f[i] = (float)d->mu[i]; sd_f[i] = (float)d->sd[i]; lin_f[i] = (float)d->lin[i]; }
     for (size_t i = 0; i < wts.size(); ++i) w_f[i] = (float)wts[i];
@@ -564,10 +735,20 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     UP(sc.data(), sc.size(), &m.shell_count);
     UP(part_shell.data(), part_shell.size(), &m.part_shell);
     if (n_part > 0) UP((const int4 *)d->partners, (size_t)n_part, &m.part);
+    UP(c0_ch.data(), c0_ch.size(), &m.c0_ch);
     UP(d->mu, (size_t)m.C0, &m.mu); UP(d->sd, (size_t)m.C0, &m.sd); UP(d->lin, (size_t)m.C0, &m.lin);
     UP(wts.data(), wts.size(), &m.wts); UP(hw.data(), hw.size(), &m.head_w);
     UP(mu_f.data(), mu_f.size(), &m.mu_f); UP(sd_f.data(), sd_f.size(), &m.sd_f); UP(lin_f.data(), lin_f.size(), &m.lin_f);
     UP(w_f.data(), w_f.size(), &m.wts_f); UP(hw_f.data(), hw_f.size(), &m.head_w_f);
+    m.pr_elems = pr_max;
+    m.hit_elems = hit_max;
+    m.w_elems = 1;
+    for (int l = 0; l < m.L; ++l)
+        m.w_elems = std::max<int64_t>(m.w_elems, (int64_t)(l == 0 ? m.c0_eff : m.cin[l]) * m.mcat[l]);
+    m.lds_mode32 = nn_lds_mode<float>(m);
+    m.lds_mode64 = nn_lds_mode<double>(m);
+    if (nn_lds_for<float>(m, 0).total > 160 * 1024 || nn_lds_for<double>(m, 0).total > 160 * 1024)
+        return set_err(NCE_ERR_UNSUPPORTED, "receptive field too large for the LDS hash table");
     c->nn_models.push_back(m);
     if (int rc = nn_sync_models(c)) return rc;
     *model_id = (int32_t)c->nn_models.size() - 1;
@@ -592,7 +773,7 @@ static int nn_full_cache_impl(nce_ctx *c, int32_t model, const int8_t *occ, int3
     if (int rc = allow_lds(nn_h0_kernel<Real>, lds0)) return rc;
     hipLaunchKernelGGL(nn_h0_kernel<Real>, dim3(nb0, R), dim3(NN_H0_SITES), lds0, s, dm, model, occ, N, cache);
     HIP_TRY(hipGetLastError());
-    for (int l = 0; l + 1 < m.L; ++l) {
+    for (int l = 0; l < m.L; ++l) {
         const int64_t n = N * m.dd[l];
         hipLaunchKernelGGL(nn_layer_kernel<Real>, dim3((unsigned)((n + 255) / 256), R), dim3(256), 0, s, dm, model,
                            l, N, cache);
@@ -636,6 +817,7 @@ extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jo
     HIP_TRY(hipSetDevice(c->device));
     std::vector<int2> items;
     size_t lds = 0;
+    int mode = 0;
     for (int j = 0; j < n_jobs; ++j) {
         const nce_nn_job &jb = jobs[j];
         if (jb.model < 0 || jb.model >= (int)c->nn_models.size()) return set_err(NCE_ERR_ARG, "unknown NeuralCE model");
@@ -648,7 +830,13 @@ extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jo
         if ((jb.inj_ia != nullptr) != (jb.inj_ib != nullptr) || (jb.inj_ia != nullptr) != (jb.inj_u != nullptr))
             return set_err(NCE_ERR_ARG, "injected streams must be all set or all NULL");
         for (int r = 0; r < jb.n_replicas; ++r) items.push_back(make_int2(j, r));
-        lds = std::max(lds, nn_lds_layout(m.hash_cap, m.ball_cap[m.L]).total);
+        const int mj = precision == NCE_PREC_FP64 ? m.lds_mode64 : m.lds_mode32;
+        mode = j == 0 ? mj : std::min(mode, mj);
+    }
+    for (int j = 0; j < n_jobs; ++j) {
+        const NNModelDev &m = c->nn_models[jobs[j].model];
+        lds = std::max(lds, precision == NCE_PREC_FP64 ? nn_lds_for<double>(m, mode).total
+                                                       : nn_lds_for<float>(m, mode).total);
     }
     if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "receptive field does not fit in LDS");
     const size_t jb_bytes = align16(sizeof(nce_nn_job) * n_jobs);
@@ -661,15 +849,24 @@ extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jo
     const nce_nn_job *d_jobs = (const nce_nn_job *)dev;
     const int2 *d_items = (const int2 *)((unsigned char *)dev + jb_bytes);
     const NNModelDev *dm = (const NNModelDev *)c->d_nn_models;
+    auto launch = [&](auto kernel) -> int {
+        if (int rc = allow_lds(kernel, lds)) return rc;
+        hipLaunchKernelGGL(kernel, dim3((unsigned)items.size()), dim3(NN_BS), lds, s, dm, d_jobs, d_items, n_steps);
+        HIP_TRY(hipGetLastError());
+        return NCE_OK;
+    };
     if (precision == NCE_PREC_FP64) {
-        if (int rc = allow_lds(nn_chunk_kernel<double>, lds)) return rc;
-        hipLaunchKernelGGL(nn_chunk_kernel<double>, dim3((unsigned)items.size()), dim3(NN_BS), lds, s, dm, d_jobs,
-                           d_items, n_steps);
-    } else {
-        if (int rc = allow_lds(nn_chunk_kernel<float>, lds)) return rc;
-        hipLaunchKernelGGL(nn_chunk_kernel<float>, dim3((unsigned)items.size()), dim3(NN_BS), lds, s, dm, d_jobs,
-                           d_items, n_steps);
+        switch (mode) {
+            case 0: return launch(nn_chunk_kernel<double, 0>);
+            case 1: return launch(nn_chunk_kernel<double, 1>);
+            case 2: return launch(nn_chunk_kernel<double, 2>);
+            default: return launch(nn_chunk_kernel<double, 3>);
+        }
     }
-    HIP_TRY(hipGetLastError());
-    return NCE_OK;
+    switch (mode) {
+        case 0: return launch(nn_chunk_kernel<float, 0>);
+        case 1: return launch(nn_chunk_kernel<float, 1>);
+        case 2: return launch(nn_chunk_kernel<float, 2>);
+        default: return launch(nn_chunk_kernel<float, 3>);
+    }
 }
