@@ -38,13 +38,22 @@ __device__ inline const Real *nn_headw(const NNModelDev &m) {
     if constexpr (sizeof(Real) == 8) return m.head_w;
     else return m.head_w_f;
 }
-template <typename Real>
-__device__ inline Real nn_sigmoid(Real x) { return (Real)1 / ((Real)1 + exp(-x)); }
+// Activations.  FP64: the C library functions.  FP32: from the hardware base-2 exponential and reciprocal
+// (v_exp_f32, v_rcp_f32, about 1 ulp each): sigmoid(x) = 1 / (1 + 2^(-x log2 e)), tanh(x) = 1 - 2 / (1 + 2^(2x log2 e));
+// the absolute error stays at a few 1e-8, the FP32 rounding level of the values themselves.
+__device__ inline double nn_sigmoid(double x) { return 1.0 / (1.0 + exp(-x)); }
+__device__ inline double nn_tanh(double x) { return tanh(x); }
+__device__ inline float nn_sigmoid(float x) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * x));
+}
+__device__ inline float nn_tanh(float x) {
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(2.8853900817779268f * x));
+}
 
 template <typename Real>
 __device__ inline Real nn_act(const NNModelDev &m, Real pre, Real gpre) {
     if (m.act == 1) return pre * nn_sigmoid(pre);
-    Real out = tanh(pre);
+    Real out = nn_tanh(pre);
     if (m.has_gate) out *= nn_sigmoid(gpre);
     return out;
 }
@@ -177,7 +186,7 @@ __global__ void nn_energy_finish_kernel(const NNModelDev *__restrict__ models, i
 }
 
 // ── local update chunk kernel ─────────────────────────────────────────────────────────────────────────────
-constexpr int NN_BS = 512;
+constexpr int NN_BS = 768;
 constexpr int NN_EMPTY = -1;
 constexpr int NN_RB = 4;                 // rows per thread in the projection
 
