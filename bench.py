@@ -40,7 +40,7 @@ CONFIGS = {
     "ce_131k": dict(n=32, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
     "ce_10m": dict(n=136, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
     # NeuralCE: 3 SiLU graph-conv layers of width 64 over the 12 nearest neighbours on 40 cluster channels
-    "nce_16k": dict(n=16, x=0.5, R=64, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed", nce="silu"),
+    "nce_16k": dict(n=16, x=0.5, R=64, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="bf16", nce="silu"),
     # NeuralCE: the reference's shipped PtNi model (depth 1, width 16, 3 shells) at 16,384 sites, 32 replicas
     "nce_ptni_16k": dict(n=16, x=0.5, R=32, Tmin=700.0, Tmax=1100.0, exchange=5000, precision="mixed", nce="ptni"),
 }
@@ -68,7 +68,7 @@ def nce_bench(args, cfg):
     R = cfg["R"]
     temps = np.geomspace(cfg["Tmin"], cfg["Tmax"], R)
     betas = (1.0 / (8.617333262e-5 * temps)).astype(np.float32).astype(np.float64)
-    prec = 1 if cfg["precision"] == "mixed" else 0
+    prec = {"fp64": 0, "mixed": 1, "bf16": 2}[cfg["precision"]]
     sig = nn.geo["sig"].astype(np.int8)
     st = NNJobState(nn, sig, betas, seed=args.seed, precision=prec)
     chunk = args.chunk
@@ -86,20 +86,27 @@ def nce_bench(args, cfg):
     dt = time.perf_counter() - t0
     k_ms = np.array([a.elapsed_time(b) for a, b in evs])
     value = R * chunk * args.steps / dt
-    # convolution FLOPs per attempt: every layer on the union of the two flips' balls (mean union size measured
-    # on the host from the lattice offsets of random pairs), 2 FLOP per multiply-add of the self + shell maps
+    # algorithmic FLOPs per attempt of the graph convolutions (the incremental local update): per hidden layer l the
+    # change of layer l-1 on ball_{l-1} (union of the two flips' balls, from the geometry) projected through the
+    # concatenated [self | shell | gate] maps, 2 FLOP per multiply-add; a swap of two equal species (probability
+    # 1 - 2x(1-x) for a random pair) does no work
     geo = nn.geo
+    caps = [min(2 * sz, nn.N) for sz in geo["ball_sizes"]]
+    n_sh = len(geo["shell_count"])
     fl = 0.0
-    sizes = geo["ball_sizes"]
-    for l, L in enumerate(P["layers"]):
-        cin, d = L["self"].shape
-        k = cin * (1 + len(geo["shell_count"])) * (2 if "gate" in L else 1)
-        fl += 2.0 * k * d * min(2 * sizes[l + 1], nn.N)
+    for l, Lr in enumerate(P["layers"]):
+        cin, d = Lr["self"].shape
+        fl += 2.0 * caps[l] * cin * d * (1 + n_sh + (n_sh if "gate" in Lr else 0))
+    fl *= 2.0 * cfg["x"] * (1.0 - cfg["x"])
     achieved = fl * R * chunk / (k_ms.mean() * 1e-3) / 1e12
-    peak = 157.3                                                   # FP32 vector (the kernel's FMA path)
+    peak = 2500.0 if prec == 2 else (157.3 if prec == 1 else 78.6)   # dense bf16 MFMA / fp32 / fp64
+    # running totals against a full recompute of the caches after the timed chunks (drift of the local update)
+    E_run = st.energy.clone()
+    st.anchor()
+    drift = float((st.energy - E_run).abs().max().item()) * (nn.ys if cfg["nce"] == "ptni" else 1.0)
     acc = float(st.n_accept.sum().item()) / (R * chunk * (args.warmup + args.steps))
     return dict(value=value, dt=dt, k_ms=k_ms, achieved=achieved, peak=peak, acc=acc, R=R, N=nn.N,
-                flops_attempt=fl, prec=prec)
+                flops_attempt=fl, prec=prec, drift=drift)
 
 
 def parse():
@@ -109,7 +116,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--chunk", type=int, default=5000, help="MC steps per replica per chunk")
     ap.add_argument("--config", default="ce_1m", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed", "bf16"])
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -174,14 +181,17 @@ def main():
         line = {"metric": "MC swap attempts/s (all replicas)", "value": res["value"], "unit": "attempts/s",
                 "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["dt"] / args.steps * 1e3,
                 "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-                "dtype": "f32" if res["prec"] else "f64", "data": "synthetic (seeded random occupations / weights)",
+                "dtype": ["f64", "f32", "bf16 mfma / f32"][res["prec"]],
+                "data": "synthetic (seeded random occupations / weights)",
                 "config": {"workload": f"NeuralCE PT swap MC ({cfg['nce']}), fcc {cfg['n']}^3 cubic = {res['N']} "
                                        f"sites, {res['R']} replicas", "chunk_steps": args.chunk},
-                "roofline": {"bound": "valu-fp32", "achieved": res["achieved"], "peak": res["peak"],
+                "roofline": {"bound": "mfma" if res["prec"] == 2 else "valu", "achieved": res["achieved"],
+                             "peak": res["peak"],
                              "unit": "TFLOP/s", "frac": res["achieved"] / res["peak"], "traffic": None,
                              "kernel": "nn_chunk_kernel", "kernel_ms_avg": float(res["k_ms"].mean()),
                              "flops_per_attempt": res["flops_attempt"]},
-                "cpu_baseline": None, "acceptance": res["acc"]}
+                "cpu_baseline": None, "acceptance": res["acc"],
+                "drift_vs_full_recompute": res["drift"]}
         print(json.dumps(line), flush=True)
         return
     prec = PREC_MIXED if cfg["precision"] == "mixed" else PREC_FP64

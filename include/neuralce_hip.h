@@ -29,6 +29,8 @@ extern "C" {
 
 #define NCE_PREC_FP64 0      /* plain form: FP64 swap energy change */
 #define NCE_PREC_MIXED 1     /* FP32 swap energy change, FP64 running totals / exchange / accumulators */
+#define NCE_PREC_BF16 2      /* NeuralCE only: as MIXED, with the graph-convolution projections on BF16 MFMA
+                                (FP32 accumulation) */
 
 typedef struct nce_ctx nce_ctx;
 

@@ -15,6 +15,7 @@ LIB_PATH = os.environ.get("NCE_HIP_LIB", os.path.join(_HERE, "libneuralce_hip.so
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_BF16 = 2          # NeuralCE: FP32 caches, graph-convolution projections on BF16 MFMA
 
 _p = C.c_void_p
 

@@ -131,6 +131,7 @@ struct NNModelDev {
     // ball 0, 2 and the sorted partner hits, 3 and the projections
     int32_t lds_mode32, lds_mode64;
     int64_t pr_elems, hit_elems, w_elems;   // largest projection block / hit list / used projection matrix
+    int32_t mfma_ok;             // every layer's projection width is a multiple of 16 (BF16 MFMA tiles)
     int64_t off_pre[NN_MAXL], off_gpre[NN_MAXL], cache_elems;
     int64_t soff_dh0, soff_pr, soff_tgt, soff_list, soff_dpre[NN_MAXL], soff_dgpre[NN_MAXL], soff_dh[2], scratch_elems;
 };

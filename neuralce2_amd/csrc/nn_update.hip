@@ -283,7 +283,10 @@ __device__ inline int nn_block_scan(int *a, int n, double *red_d) {
     return total;
 }
 
-template <typename Real, int MODE>
+typedef __bf16 nn_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float nn_floatx4 __attribute__((ext_vector_type(4)));
+
+template <typename Real, int MODE, bool MFMA>
 __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__restrict__ models,
                                                          const nce_nn_job *__restrict__ jobs,
                                                          const int2 *__restrict__ items, int64_t n_steps) {
@@ -393,8 +396,57 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const Real *dx = l == 0 ? dh0 : scr + m.soff_dh[(l - 1) & 1];
                 const int xs = l == 0 ? C0 : m.cin[l];
                 const Real *Wc = W + m.off_wcat[l];
-                // (1) pr[q][j] = dx[q] . Wcat[:, j]: the layer's matrix (used rows only) staged in LDS, 4 x 4
-                // register tiles
+                // (1) pr[q][j] = dx[q] . Wcat[:, j]
+                if constexpr (MFMA) {
+                    // BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation): the layer's matrix staged in LDS as
+                    // bf16, transposed [j][k] with k padded to 32; one 16 x 16 output tile per wave at a time.
+                    // Lane l holds A[row l%16][k 8(l/16)..+7], B[k 8(l/16)..+7][col l%16], C[rows 4(l/16)..+3][col l%16].
+                    const int Kp = (Cin + 31) & ~31;
+                    __bf16 *wt = (__bf16 *)wl;
+                    for (int k = tid; k < M * Kp; k += NN_BS) {
+                        const int j = k / Kp, c = k - j * Kp;
+                        wt[k] = c < Cin ? (__bf16)Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j] : (__bf16)0.0f;
+                    }
+                    __syncthreads();
+                    const int lane = tid & 63, wv = tid >> 6;
+                    const int nrt = (prev + 15) >> 4, nct = M >> 4;
+                    constexpr int CT = 4;                // column tiles per pass (A fragment loaded once per pass)
+                    for (int task = wv; task < nrt * ((nct + CT - 1) / CT); task += NN_BS / 64) {
+                        const int rt = task % nrt, cg = (task / nrt) * CT;
+                        const int row = rt * 16 + (lane & 15), kb = 8 * (lane >> 4);
+                        nn_floatx4 acc[CT];
+#pragma unroll
+                        for (int t = 0; t < CT; ++t) acc[t] = nn_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                        for (int k0 = 0; k0 < Kp; k0 += 32) {
+                            nn_bf16x8 av;
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                const int c = k0 + kb + i;
+                                float x = 0.0f;
+                                if (row < prev && c < Cin) x = (float)dx[(int64_t)row * xs + (l == 0 ? m.c0_ch[c] : c)];
+                                av[i] = (__bf16)x;
+                            }
+#pragma unroll
+                            for (int t = 0; t < CT; ++t) {
+                                if (cg + t < nct) {
+                                    const nn_bf16x8 bv =
+                                        *(const nn_bf16x8 *)(wt + ((cg + t) * 16 + (lane & 15)) * Kp + k0 + kb);
+                                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < CT; ++t) {
+                            if (cg + t >= nct) continue;
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                const int r0 = rt * 16 + 4 * (lane >> 4) + i;
+                                if (r0 < prev) pr[(int64_t)r0 * M + (cg + t) * 16 + (lane & 15)] = (Real)acc[t][i];
+                            }
+                        }
+                    }
+                } else {
+                // the layer's matrix (used rows only) staged in LDS, 4 x 4 register tiles
                 for (int k = tid; k < Cin * M; k += NN_BS) {
                     const int c = k / M, j = k - c * M;
                     wl[k] = Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j];
@@ -426,6 +478,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
 #pragma unroll
                         for (int u = 0; u < 4; ++u)
                             if (q0 + i < prev && j0 + u < M) pr[(int64_t)(q0 + i) * M + j0 + u] = acc[i][u];
+                }
                 }
                 // (2) partner sites of ball_l: claim (new sites get the next slots), then slot of every hit
                 const int nh = prev * n_part;
@@ -749,11 +802,17 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     UP(wts.data(), wts.size(), &m.wts); UP(hw.data(), hw.size(), &m.head_w);
     UP(mu_f.data(), mu_f.size(), &m.mu_f); UP(sd_f.data(), sd_f.size(), &m.sd_f); UP(lin_f.data(), lin_f.size(), &m.lin_f);
     UP(w_f.data(), w_f.size(), &m.wts_f); UP(hw_f.data(), hw_f.size(), &m.head_w_f);
+    if (pr_max >= ((int64_t)1 << 31)) return set_err(NCE_ERR_UNSUPPORTED, "projection block too large");
     m.pr_elems = pr_max;
     m.hit_elems = hit_max;
     m.w_elems = 1;
-    for (int l = 0; l < m.L; ++l)
-        m.w_elems = std::max<int64_t>(m.w_elems, (int64_t)(l == 0 ? m.c0_eff : m.cin[l]) * m.mcat[l]);
+    m.mfma_ok = m.L > 0;
+    for (int l = 0; l < m.L; ++l) {
+        const int64_t cin = l == 0 ? m.c0_eff : m.cin[l];
+        m.w_elems = std::max<int64_t>(m.w_elems, cin * m.mcat[l]);
+        m.w_elems = std::max<int64_t>(m.w_elems, (((cin + 31) & ~31) * m.mcat[l] + 1) / 2);   // bf16 copy
+        if (m.mcat[l] % 16 != 0) m.mfma_ok = 0;
+    }
     m.lds_mode32 = nn_lds_mode<float>(m);
     m.lds_mode64 = nn_lds_mode<double>(m);
     if (nn_lds_for<float>(m, 0).total > 160 * 1024 || nn_lds_for<double>(m, 0).total > 160 * 1024)
@@ -813,7 +872,8 @@ extern "C" int nce_nn_full_cache(nce_ctx *c, int32_t model, const int8_t *occ, i
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t s = (hipStream_t)stream;
     if (precision == NCE_PREC_FP64) return nn_full_cache_impl<double>(c, model, occ, R, N, (double *)cache, energy, s);
-    if (precision == NCE_PREC_MIXED) return nn_full_cache_impl<float>(c, model, occ, R, N, (float *)cache, energy, s);
+    if (precision == NCE_PREC_MIXED || precision == NCE_PREC_BF16)
+        return nn_full_cache_impl<float>(c, model, occ, R, N, (float *)cache, energy, s);
     return set_err(NCE_ERR_ARG, "unknown precision");
 }
 
@@ -821,12 +881,14 @@ extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jo
                                 int32_t precision, void *stream) {
     if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
     if (n_steps < 0) return set_err(NCE_ERR_ARG, "negative n_steps");
-    if (precision != NCE_PREC_FP64 && precision != NCE_PREC_MIXED) return set_err(NCE_ERR_ARG, "unknown precision");
+    if (precision != NCE_PREC_FP64 && precision != NCE_PREC_MIXED && precision != NCE_PREC_BF16)
+        return set_err(NCE_ERR_ARG, "unknown precision");
     if (n_steps == 0) return NCE_OK;
     HIP_TRY(hipSetDevice(c->device));
     std::vector<int2> items;
     size_t lds = 0;
     int mode = 0;
+    bool mfma = precision == NCE_PREC_BF16;
     for (int j = 0; j < n_jobs; ++j) {
         const nce_nn_job &jb = jobs[j];
         if (jb.model < 0 || jb.model >= (int)c->nn_models.size()) return set_err(NCE_ERR_ARG, "unknown NeuralCE model");
@@ -840,6 +902,7 @@ extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jo
             return set_err(NCE_ERR_ARG, "injected streams must be all set or all NULL");
         for (int r = 0; r < jb.n_replicas; ++r) items.push_back(make_int2(j, r));
         const int mj = precision == NCE_PREC_FP64 ? m.lds_mode64 : m.lds_mode32;
+        mfma = mfma && m.mfma_ok;
         mode = j == 0 ? mj : std::min(mode, mj);
     }
     for (int j = 0; j < n_jobs; ++j) {
@@ -866,16 +929,24 @@ extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jo
     };
     if (precision == NCE_PREC_FP64) {
         switch (mode) {
-            case 0: return launch(nn_chunk_kernel<double, 0>);
-            case 1: return launch(nn_chunk_kernel<double, 1>);
-            case 2: return launch(nn_chunk_kernel<double, 2>);
-            default: return launch(nn_chunk_kernel<double, 3>);
+            case 0: return launch(nn_chunk_kernel<double, 0, false>);
+            case 1: return launch(nn_chunk_kernel<double, 1, false>);
+            case 2: return launch(nn_chunk_kernel<double, 2, false>);
+            default: return launch(nn_chunk_kernel<double, 3, false>);
+        }
+    }
+    if (mfma) {
+        switch (mode) {
+            case 0: return launch(nn_chunk_kernel<float, 0, true>);
+            case 1: return launch(nn_chunk_kernel<float, 1, true>);
+            case 2: return launch(nn_chunk_kernel<float, 2, true>);
+            default: return launch(nn_chunk_kernel<float, 3, true>);
         }
     }
     switch (mode) {
-        case 0: return launch(nn_chunk_kernel<float, 0>);
-        case 1: return launch(nn_chunk_kernel<float, 1>);
-        case 2: return launch(nn_chunk_kernel<float, 2>);
-        default: return launch(nn_chunk_kernel<float, 3>);
+        case 0: return launch(nn_chunk_kernel<float, 0, false>);
+        case 1: return launch(nn_chunk_kernel<float, 1, false>);
+        case 2: return launch(nn_chunk_kernel<float, 2, false>);
+        default: return launch(nn_chunk_kernel<float, 3, false>);
     }
 }

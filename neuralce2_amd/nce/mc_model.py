@@ -83,7 +83,7 @@ def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0):
     P, mu, sd = expand_params(mdl, seed_idx)
     nn = NNModel(cs, atoms, P, mu, sd, list(mdl["shell_edges"]), act="gate", readout="mean",
                  ys=mdl["ys"], ym=mdl["ym"], device=device)
-    prec = 0 if dtype in ("float64", None) else 1
+    prec = {"float64": 0, None: 0, "bfloat16": 2, "bf16": 2}.get(dtype, 1)
     geo = nn.geo
 
     def energy(sig, spin_state=None):

@@ -150,3 +150,64 @@ def test_silu_three_layer_network(torch_dev):
         o.run_chunk(occ_ref, E, betas, (perm[ia], perm[ib], u), 1.0)
         assert np.array_equal(st.occ.cpu().numpy(), occ_ref[:, perm])
         assert np.max(np.abs(st.energy.cpu().numpy() - E)) < (1e-9 if prec == 0 else 1e-3)
+
+
+@pytest.mark.parametrize("kind", ["silu", "ptni"])
+def test_bf16_mfma_energy_change(torch_dev, ptni_nce, kind):
+    """BF16 MFMA projections (FP32 accumulation, FP32 caches): the energy change of one accepted swap per replica
+    against the FP64 oracle's difference of full energies, within 3% of |dE| + 1e-5 (model units per site for the
+    mean readout, totals for the summed one); after a chain of such steps the running total matches a full recompute
+    within the same bound per step."""
+    import torch
+    from neuralce2_amd.nce.mc_model import NNModel, silu_network
+    from neuralce2_amd.nce.mcmc import NNJobState
+    from neuralce2_amd.synthetic import fcc_cluster_space
+    from oracle.nce_oracle import Oracle
+    rng = np.random.default_rng(7)
+    if kind == "silu":
+        cs = fcc_cluster_space((9.0, 6.0, 5.0))
+        P = silu_network(41, 64, 3, 1, seed=3)
+        mu, sd = np.zeros(41), np.ones(41)
+        at = _cell("5", 1)
+        edges = [0.0, 3.3]
+        nn = NNModel(cs, at, P, mu, sd, edges, act="silu", readout="sum")
+        o = Oracle(cs, at, edges, P, mu, sd, kind="silu", readout="sum")
+    else:
+        at = _cell("4", 1)
+        nn, o = _models(ptni_nce, at)
+    perm, N = nn.geo["perm"], nn.N
+    R = 16
+    sig_o = (at.numbers == 78).astype(np.int8)
+    occ_o = np.stack([rng.permutation(sig_o) for _ in range(R)])
+    betas = np.full(R, 1.0 / (8.617333262e-5 * 1000.0))
+    st = NNJobState(nn, occ_o[:, perm], betas, precision=2, sample_interval=1, samples_per_chunk=1)
+    E0 = st.energy.cpu().numpy().copy()
+    ia = np.zeros((R, 1), np.int64); ib = np.zeros((R, 1), np.int64)
+    for r in range(R):                                   # a real swap (two different species) per replica
+        a = int(rng.integers(N)); b = int(rng.choice(np.flatnonzero(occ_o[r, perm] != occ_o[r, perm][a])))
+        ia[r, 0], ib[r, 0] = a, b
+    u = np.zeros((R, 1))                                 # u = 0: every swap is accepted
+    inj = tuple(torch.tensor(x, device=torch_dev) for x in (ia, ib, u))
+    st.run(1, inj)
+    torch.cuda.synchronize()
+    dE = st.energy.cpu().numpy() - E0
+    ref = []
+    for r in range(R):
+        s1 = occ_o[r].copy(); a, b = perm[ia[r, 0]], perm[ib[r, 0]]
+        s1[a], s1[b] = s1[b], s1[a]
+        ref.append(o.energy(s1) - o.energy(occ_o[r]))
+    ref = np.array(ref)
+    err = np.abs(dE - ref)
+    assert np.all(err <= 3e-2 * np.abs(ref) + 1e-5), (err, ref)
+    # chain: running total against a full recompute of the caches
+    steps = 30
+    ia = rng.integers(0, N, (R, steps)); ib = rng.integers(0, N, (R, steps)); u = rng.random((R, steps))
+    st2 = NNJobState(nn, occ_o[:, perm], betas, precision=2, sample_interval=1, samples_per_chunk=steps)
+    st2.run(steps, tuple(torch.tensor(x, device=torch_dev) for x in (ia, ib, u)))
+    torch.cuda.synchronize()
+    E_run = st2.energy.cpu().numpy().copy()
+    st2.anchor()
+    E_full = st2.energy.cpu().numpy()
+    scale = np.max(np.abs(ref)) + 1e-5
+    assert np.max(np.abs(E_run - E_full)) <= steps * 3e-2 * scale, (np.abs(E_run - E_full).max(), scale)
+    print(kind, "max |dE err| / |dE|", float(np.max(err / (np.abs(ref) + 1e-12))), "chain drift", float(np.abs(E_run - E_full).max()))
