@@ -109,6 +109,21 @@ def nce_bench(args, cfg):
                 flops_attempt=fl, prec=prec, drift=drift)
 
 
+def served_bench(args):
+    """Served request stream: Poisson arrivals of C_v(T) requests, continuous batching on one GPU, against the plain
+    FP64 one-job-at-a-time form on the same requests."""
+    from neuralce2_amd.serve import Server, job_stream, run_sequential
+    specs = job_stream(args.jobs, args.rate, seed=args.seed, n_min=4, n_max=args.n_max, t_min=16, t_max=64,
+                       n_steps=args.job_steps, chunk=args.chunk)
+    srv = Server(chunk=args.chunk, precision=args.precision or "mixed")
+    srv.run(specs[:2])                                   # warm-up: code objects, model tables, tile cache
+    res, st = srv.run(specs)
+    base = None
+    if not args.no_cpu:                                  # the plain form is the comparison; skipped with --no-cpu
+        _, base = run_sequential(specs, precision="fp64", chunk=args.chunk)
+    return res, st, base, specs
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +136,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU time of the cpu_baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--served", action="store_true", help="served request stream (p50/p99 job latency)")
+    ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=100.0, help="requests per second (Poisson)")
+    ap.add_argument("--job-steps", type=int, default=100000)
+    ap.add_argument("--n-max", type=int, default=32)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 1 GiB (the known-size kernel that scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -167,6 +187,23 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.served:
+        res, st, base, specs = served_bench(args)
+        line = {"metric": "MC swap attempts/s (all replicas, served stream); p50/p99 PT-job latency",
+                "value": st["attempts_per_s"], "unit": "attempts/s", "n_gpus": 1, "steps": st["n_ticks"],
+                "warmup": 1, "ms_per_step": st["wall_s"] / max(st["n_ticks"], 1) * 1e3, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32 dE / f64 totals",
+                "data": "synthetic request stream (seeded)",
+                "config": {"workload": f"{args.jobs} C_v(T) requests, Poisson {args.rate}/s, fcc n^3 cubic cells "
+                                       f"log-uniform 256-{4 * args.n_max ** 3} sites, x in 0.1..0.9, 16-64 "
+                                       f"temperatures, {args.job_steps} steps each, chunk {args.chunk}"},
+                "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"],
+                "wall_s": st["wall_s"], "attempts": st["attempts"],
+                "plain_fp64_sequential": base,
+                "speedup_vs_plain_fp64_sequential": (st["attempts_per_s"] / base["attempts_per_s"]) if base else None,
+                "roofline": None, "cpu_baseline": None}
+        print(json.dumps(line), flush=True)
         return
     import torch
     from neuralce2_amd.synthetic import synthetic_ce, fcc_cell

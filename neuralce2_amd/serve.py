@@ -1,0 +1,364 @@
+"""Served parallel-tempering jobs on one GPU: many PT jobs of uneven cell size, composition and temperature ladder.
+
+A request is a C_v(T) curve: a cell (fcc n x n x n conventional cells, Pt fraction x), a temperature ladder and a number
+of MC steps per replica.  The server keeps every admitted job's replica state resident in device memory and runs
+continuous batching at chunk granularity:
+
+  * admission: a request that has arrived is admitted when its device memory (occupations, caches, counters) fits in
+    the budget (default: the smaller of 288 GB and 90 % of the free device memory); requests are taken in arrival
+    order;
+  * one tick = one chunk of `chunk` steps of every admitted cluster-expansion job in ONE nce_pt_run_chunk launch
+    (all (job, replica) pairs; the speculative kernel handles jobs of different cells together) and one chunk of
+    every admitted NeuralCE job in one nce_nn_run_chunk launch on a second stream, then the temperature-swap
+    exchange of every CE job in one nce_pt_exchange launch;
+  * a job that has done its steps retires: its per-slot <E> and <E^2> (FP64, every production step) give E(T) and
+    C_v(T), its memory is released, and waiting requests are admitted before the next tick.
+
+Random numbers are Philox keyed by (job seed, replica, step) and the exchange by (job seed, exchange count), so a
+cluster-expansion job's trajectory does not depend on which other jobs share its launches: a job gives the same result
+alone and in the mix (the speculative kernel's decisions are the sequential chain's, whatever the tile).  NeuralCE
+jobs match within the FP32 rounding of their sums (their per-step slot numbering comes from LDS atomics).
+
+Job latency = retirement time - arrival time; the server reports p50 / p99 and the aggregate swap attempts/s.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+KB_CE = 8.617333e-5
+
+
+@dataclass
+class JobSpec:
+    job_id: int
+    n: int                      # fcc n x n x n conventional cells: 4 n^3 sites
+    x: float                    # Pt fraction
+    n_temps: int
+    n_steps: int
+    T_min: float = 300.0
+    T_max: float = 1400.0
+    seed: int = 0
+    arrival: float = 0.0        # seconds after the start of the stream
+    kind: str = "ce"            # "ce" (synthetic 8/6/5 cluster expansion) | "nce" (NeuralCE model of the server)
+
+
+@dataclass
+class JobResult:
+    job_id: int
+    kind: str
+    n_sites: int
+    temps: np.ndarray
+    E_mean: np.ndarray          # eV / site, per temperature
+    C_v: np.ndarray             # k_B / site, per temperature
+    attempts: int
+    acceptance: float
+    arrival: float
+    admitted: float
+    finished: float
+    occ: Optional[np.ndarray] = None   # final occupations by temperature slot (canonical order), on request
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def _ladder(spec):
+    temps = np.array([spec.T_min]) if spec.n_temps == 1 else np.geomspace(spec.T_min, spec.T_max, spec.n_temps)
+    return temps.astype(np.float32), (1.0 / (KB_CE * temps)).astype(np.float32).astype(np.float64)
+
+
+def _initial_sigma(n_sites, x, seed):
+    """random occupation with round(x N) Pt sites (species index 1), canonical order, from the job seed."""
+    rng = np.random.default_rng(seed)
+    sig = np.zeros(n_sites, np.int8)
+    sig[rng.choice(n_sites, int(round(x * n_sites)), replace=False)] = 1
+    return sig
+
+
+class _CEJob:
+    def __init__(self, spec, gce, chunk):
+        from .ce.pt_mcmc_ce import PTJobState
+        self.spec = spec
+        self.temps, betas = _ladder(spec)
+        self.N = gce.n_atoms
+        self.state = PTJobState(gce, _initial_sigma(self.N, spec.x, spec.seed), betas, seed=spec.seed,
+                                sample_interval=chunk, samples_per_chunk=1, job_id=spec.job_id)
+        self.steps_done = 0
+        self.n_ex = 0
+
+    @staticmethod
+    def bytes_needed(spec):
+        N = 4 * spec.n ** 3
+        return spec.n_temps * (N + 128) + 4096
+
+    def result(self, keep_occ):
+        st = self.state
+        mean, var = st.moments()                      # per slot, total energies
+        T = self.temps.astype(np.float64)
+        E = mean / self.N
+        cv = var / (KB_CE * T) ** 2 / self.N
+        acc = float(st.n_accept.sum().item()) / max(1, self.steps_done * st.R)
+        occ = None
+        if keep_occ:
+            rep = st.rep_of_slot.cpu().numpy()
+            occ = st.occ.cpu().numpy()[rep]
+        return E, cv, acc, occ
+
+
+class _NCEJob:
+    """NeuralCE job: FP32 (or BF16-MFMA) local updates; the exchange permutes configurations on the host
+    (neuralce/nce/mcmc.py:370-384), every chunk."""
+
+    def __init__(self, spec, nn, chunk, precision):
+        from .nce.mcmc import NNJobState
+        self.spec = spec
+        self.nn = nn
+        self.temps, betas = _ladder(spec)
+        self.betas = (1.0 / (8.617333262e-5 * self.temps.astype(np.float64))).astype(np.float32).astype(np.float64)
+        self.N = nn.N
+        self.state = NNJobState(nn, _initial_sigma(self.N, spec.x, spec.seed), self.betas, seed=spec.seed,
+                                precision=precision, sample_interval=1, samples_per_chunk=chunk)
+        self.rng = np.random.default_rng(spec.seed)
+        self.steps_done = 0
+        self.acc_n = 0
+        self.s1 = np.zeros(spec.n_temps)
+        self.s2 = np.zeros(spec.n_temps)
+
+    @staticmethod
+    def bytes_needed(spec, nn, precision):
+        el = 8 if precision == 0 else 4
+        return spec.n_temps * (nn.N + (nn.cache_elems + nn.scratch_elems) * el + 128) + 4096
+
+    def after_chunk(self, chunk):
+        tr = self.state.traj.cpu().numpy()            # (chunk, R) model units, replica = temperature
+        e = tr * self.nn.ys + self.nn.ym if self.nn.readout == "mean" else tr / self.N
+        self.s1 += e.sum(0)
+        self.s2 += (e * e).sum(0)
+        self.acc_n += e.shape[0]
+        R = self.state.R
+        start = int(self.rng.integers(0, 2))
+        u = self.rng.random(R)
+        E_tot = self.state.energy.cpu().numpy() * self.nn.de_scale
+        order = list(range(R))
+        for i in range(start, R - 1, 2):
+            d = (self.betas[i] - self.betas[i + 1]) * (E_tot[i] - E_tot[i + 1])
+            if d >= 0 or u[i] < math.exp(min(d, 0.0)):
+                order[i], order[i + 1] = order[i + 1], order[i]
+        if order != list(range(R)):
+            self.state.permute(order)
+
+    def result(self, keep_occ):
+        n = max(self.acc_n, 1)
+        mean = self.s1 / n
+        var = np.maximum(self.s2 / n - mean * mean, 0.0)
+        T = self.temps.astype(np.float64)
+        cv = var * self.N / (8.617333262e-5 * T) ** 2
+        acc = float(self.state.n_accept.sum().item()) / max(1, self.steps_done * self.state.R)
+        occ = self.state.occ.cpu().numpy() if keep_occ else None
+        return mean, cv, acc, occ
+
+
+def autotune_tile(ctx, states, precision, candidates=(4, 8, 16), n_steps=256):
+    """Pick the speculative kernel's tile (steps of a replica evaluated in parallel) for this set of CE jobs by timing
+    one short chunk per candidate on copies of the jobs' state (the jobs themselves do not move)."""
+    import torch
+    from .ce.pt_mcmc_ce import PTJobState
+    copies = []
+    for st in states:
+        c = PTJobState.__new__(PTJobState)
+        c.__dict__.update(st.__dict__)
+        for name in ("occ", "energy", "beta_rep", "slot_of_rep", "step", "n_accept", "e_min", "e_ref", "s1", "s2"):
+            setattr(c, name, getattr(st, name).clone())
+        c.traj = None
+        copies.append(c)
+    best, best_t = candidates[0], float("inf")
+    for tile in candidates:
+        jobs = [c.job(accumulate=False) for c in copies]
+        ctx.pt_run_chunk(jobs, n_steps, precision, tile)          # warm-up (code objects, caches)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ctx.pt_run_chunk(jobs, n_steps, precision, tile)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        if dt < best_t:
+            best, best_t = tile, dt
+    return best
+
+
+class Server:
+    """Continuous-batching PT server (see the module docstring)."""
+
+    def __init__(self, chunk=2000, nn_chunk=None, precision="mixed", nn_precision="mixed", mem_budget=None,
+                 tile=0, autotune=True, nce_model=None, ce_seed=0, keep_occ=False, device=0):
+        import torch
+        from ._lib import Context
+        if not torch.cuda.is_available():
+            raise RuntimeError("the served PT loop needs a ROCm GPU (libneuralce_hip.so); none is visible")
+        self.chunk = int(chunk)
+        self.nn_chunk = int(nn_chunk or chunk)
+        self.precision = {"fp64": 0, "mixed": 1}[precision]
+        self.nn_precision = {"fp64": 0, "mixed": 1, "bf16": 2}[nn_precision]
+        self.device = device
+        self.ctx = Context.get(device)
+        free, total = torch.cuda.mem_get_info(device)
+        self.mem_budget = int(mem_budget) if mem_budget else int(min(288e9, 0.9 * free))
+        self.tile = int(tile)
+        self.autotune = autotune and tile == 0
+        self.nce_model = nce_model          # callable(n) -> NNModel, for "nce" jobs
+        self.ce_seed = ce_seed
+        self.keep_occ = keep_occ
+        self._ce_models = {}
+        self._nn_models = {}
+        self._tile_cache = {}
+        self.stream_ce = torch.cuda.Stream(device)
+        self.stream_nn = torch.cuda.Stream(device)
+
+    def _gce(self, n):
+        if n not in self._ce_models:
+            from .ce.classical_ce_gpu import GPUClusterExpansion
+            from .synthetic import synthetic_ce, fcc_cell
+            at = fcc_cell(n, 0.5, 0)
+            self._ce_models[n] = GPUClusterExpansion(synthetic_ce(self.ce_seed), at, energy_scale=float(len(at)),
+                                                     device=self.device)
+        return self._ce_models[n]
+
+    def _nn(self, n):
+        if n not in self._nn_models:
+            if self.nce_model is None:
+                raise ValueError("NeuralCE jobs need Server(nce_model=...)")
+            self._nn_models[n] = self.nce_model(n)
+        return self._nn_models[n]
+
+    def _bytes(self, spec):
+        if spec.kind == "ce":
+            return _CEJob.bytes_needed(spec)
+        return _NCEJob.bytes_needed(spec, self._nn(spec.n), self.nn_precision)
+
+    def _tile_for(self, ce_jobs):
+        if not self.autotune:
+            return self.tile
+        import torch
+        n_items = sum(j.state.R for j in ce_jobs)
+        n_cu = torch.cuda.get_device_properties(self.device).multi_processor_count
+        key = (min(4, (n_items + n_cu - 1) // n_cu), self.precision)
+        if key not in self._tile_cache:
+            self._tile_cache[key] = autotune_tile(self.ctx, [j.state for j in ce_jobs], self.precision)
+        return self._tile_cache[key]
+
+    def run(self, specs: List[JobSpec], log=None):
+        """Serve the request stream; returns (results sorted by job id, stats)."""
+        import torch
+        pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
+        for s in pending:
+            ch = self.chunk if s.kind == "ce" else self.nn_chunk
+            if s.n_steps % ch:
+                raise ValueError(f"job {s.job_id}: n_steps {s.n_steps} is not a multiple of the chunk {ch}")
+            if self._bytes(s) > self.mem_budget:
+                raise ValueError(f"job {s.job_id} needs {self._bytes(s)} bytes, more than the budget")
+        active_ce, active_nn = [], []
+        results = []
+        mem_used = 0
+        attempts = 0
+        t0 = time.perf_counter()
+        n_ticks = 0
+        while pending or active_ce or active_nn:
+            now = time.perf_counter() - t0
+            # admission in arrival order while the memory fits
+            while pending and pending[0].arrival <= now and mem_used + self._bytes(pending[0]) <= self.mem_budget:
+                spec = pending.pop(0)
+                with torch.cuda.stream(self.stream_ce if spec.kind == "ce" else self.stream_nn):
+                    job = (_CEJob(spec, self._gce(spec.n), self.chunk) if spec.kind == "ce"
+                           else _NCEJob(spec, self._nn(spec.n), self.nn_chunk, self.nn_precision))
+                job.admitted = time.perf_counter() - t0
+                job.mem = self._bytes(spec)
+                mem_used += job.mem
+                (active_ce if spec.kind == "ce" else active_nn).append(job)
+            if not active_ce and not active_nn:
+                if pending:
+                    time.sleep(max(0.0, min(0.05, pending[0].arrival - (time.perf_counter() - t0))))
+                continue
+            torch.cuda.synchronize(self.device)
+            # one tick: every CE job in one launch, every NeuralCE job in one launch on the second stream
+            if active_ce:
+                tile = self._tile_for(active_ce)
+                with torch.cuda.stream(self.stream_ce):
+                    self.ctx.pt_run_chunk([j.state.job(accumulate=True) for j in active_ce], self.chunk,
+                                          self.precision, tile)
+            if active_nn:
+                with torch.cuda.stream(self.stream_nn):
+                    self.ctx.nn_run_chunk([j.state.job() for j in active_nn], self.nn_chunk, self.nn_precision)
+            torch.cuda.synchronize(self.device)
+            n_ticks += 1
+            if active_ce:
+                for j in active_ce:
+                    j.state.merge_accumulators(self.chunk, j.state.slot_of_rep.cpu().numpy())
+                    j.steps_done += self.chunk
+                    attempts += self.chunk * j.state.R
+                with torch.cuda.stream(self.stream_ce):
+                    self.ctx.pt_exchange([j.state.exchange_job(j.n_ex + 1, reorder_last=True) for j in active_ce])
+                for j in active_ce:
+                    j.n_ex += 1
+            for j in active_nn:
+                with torch.cuda.stream(self.stream_nn):
+                    j.after_chunk(self.nn_chunk)
+                j.steps_done += self.nn_chunk
+                attempts += self.nn_chunk * j.state.R
+            torch.cuda.synchronize(self.device)
+            # retirement
+            for group in (active_ce, active_nn):
+                done = [j for j in group if j.steps_done >= j.spec.n_steps]
+                for j in done:
+                    E, cv, acc, occ = j.result(self.keep_occ)
+                    fin = time.perf_counter() - t0
+                    results.append(JobResult(j.spec.job_id, j.spec.kind, j.N, j.temps.astype(np.float64), E, cv,
+                                             j.steps_done * j.state.R, acc, j.spec.arrival, j.admitted, fin, occ))
+                    mem_used -= j.mem
+                    group.remove(j)
+                    if log:
+                        log(f"job {j.spec.job_id} ({j.spec.kind}, {j.N} sites, {j.state.R} T) done at {fin:.2f} s, "
+                            f"latency {fin - j.spec.arrival:.2f} s")
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in results])
+        stats = dict(wall_s=wall, attempts=int(attempts), attempts_per_s=attempts / wall, n_jobs=len(results),
+                     n_ticks=n_ticks, latency_p50_s=float(np.percentile(lat, 50)) if len(lat) else None,
+                     latency_p99_s=float(np.percentile(lat, 99)) if len(lat) else None)
+        return sorted(results, key=lambda r: r.job_id), stats
+
+
+def job_stream(n_jobs, rate, seed=0, n_min=4, n_max=32, t_min=16, t_max=64, n_steps=20000, chunk=2000,
+               nce_every=0):
+    """Synthetic request stream: Poisson arrivals at `rate` per second, cells log-uniform in sites between the n_min
+    and n_max cubes, Pt fraction uniform in {0.1, ..., 0.9}, temperatures uniform in [t_min, t_max]; every
+    `nce_every`-th request a NeuralCE job (0: none).  Seeded."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n_jobs):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        lo, hi = math.log(4 * n_min ** 3), math.log(4 * n_max ** 3)
+        n = int(round((math.exp(rng.uniform(lo, hi)) / 4) ** (1.0 / 3.0)))
+        n = min(max(n, n_min), n_max)
+        x = float(rng.integers(1, 10)) / 10.0
+        nt = int(rng.integers(t_min, t_max + 1))
+        kind = "nce" if (nce_every and i % nce_every == nce_every - 1) else "ce"
+        steps = int(math.ceil(n_steps / chunk) * chunk)
+        out.append(JobSpec(job_id=i, n=n, x=x, n_temps=nt, n_steps=steps, seed=1000 + i, arrival=t, kind=kind))
+    return out
+
+
+def run_sequential(specs, precision="fp64", chunk=2000, tile=0, keep_occ=False, nce_model=None):
+    """The plain reference form: one job at a time (FP64 by default), same kernels, same per-job schedule."""
+    srv = Server(chunk=chunk, precision=precision, tile=tile, autotune=False, keep_occ=keep_occ, nce_model=nce_model)
+    res, wall, att = [], 0.0, 0
+    for s in sorted(specs, key=lambda s: s.job_id):
+        s0 = JobSpec(**{**s.__dict__, "arrival": 0.0})
+        r, st = srv.run([s0])
+        res += r
+        wall += st["wall_s"]
+        att += st["attempts"]
+    return res, dict(wall_s=wall, attempts=att, attempts_per_s=att / wall if wall else None)

@@ -1,0 +1,85 @@
+"""Served PT jobs (continuous batching of many jobs per launch): each job's result is the same alone and in the mix.
+
+Cluster-expansion jobs: bit-identical final occupations and per-temperature <E>, C_v (Philox streams keyed by job
+seed, replica and step; the speculative kernel's decisions are the sequential chain's whatever the tile).  NeuralCE
+jobs: <E>(T) within 1e-6 eV/atom (their energy sums are FP32 with a per-step order that depends on LDS atomics)."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def _ptni_factory():
+    from neuralce2_amd.nce.model_io import load_model, expand_params
+    from neuralce2_amd.clusterspace import ClusterSpace
+    from neuralce2_amd.nce.mc_model import NNModel
+    from neuralce2_amd.synthetic import fcc_cell
+    mdl = load_model(os.path.join(GOLDEN, "nipt_v3_nomax_sel.pkl"))
+    cs = ClusterSpace(mdl["primitive"], mdl["cutoffs"], mdl["chemical_symbols"])
+    P, mu, sd = expand_params(mdl, 0)
+
+    def make(n):
+        return NNModel(cs, fcc_cell(n, 0.5, 0), P, mu, sd, mdl["shell_edges"], ys=mdl["ys"], ym=mdl["ym"])
+    return make
+
+
+def test_results_same_alone_and_in_the_mix(torch_dev):
+    from neuralce2_amd.serve import Server, JobSpec
+    specs = [JobSpec(0, n=4, x=0.5, n_temps=8, n_steps=3000, seed=11),
+             JobSpec(1, n=6, x=0.25, n_temps=12, n_steps=2000, seed=12, arrival=0.0),
+             JobSpec(2, n=3, x=0.75, n_temps=6, n_steps=4000, seed=13),
+             JobSpec(3, n=8, x=0.4, n_temps=16, n_steps=2000, seed=14),
+             JobSpec(4, n=3, x=0.5, n_temps=4, n_steps=2000, seed=15, kind="nce"),
+             JobSpec(5, n=5, x=0.3, n_temps=10, n_steps=1000, seed=16)]
+    srv = Server(chunk=1000, keep_occ=True, nce_model=_ptni_factory())
+    mix, stats = srv.run(specs)
+    assert stats["n_jobs"] == len(specs) and stats["attempts"] == sum(s.n_steps * s.n_temps for s in specs)
+    for s, r in zip(specs, mix):
+        alone, _ = srv.run([s])
+        a = alone[0]
+        assert r.job_id == a.job_id == s.job_id
+        if s.kind == "ce":
+            assert np.array_equal(r.occ, a.occ), s.job_id
+            assert np.array_equal(r.E_mean, a.E_mean) and np.array_equal(r.C_v, a.C_v), s.job_id
+        else:
+            assert np.max(np.abs(r.E_mean - a.E_mean)) < 1e-6, (r.E_mean, a.E_mean)
+        assert np.all(np.isfinite(r.C_v)) and np.all(r.C_v >= 0)
+        # <E>(T) ordered with temperature on average, physical per-site energies
+        assert r.E_mean[-1] >= r.E_mean[0] - 1e-3
+
+
+def test_tile_autotuner_picks_a_candidate_without_moving_the_jobs(torch_dev):
+    import torch
+    from neuralce2_amd.serve import Server, JobSpec, _CEJob, autotune_tile
+    srv = Server(chunk=500)
+    job = _CEJob(JobSpec(0, n=4, x=0.5, n_temps=8, n_steps=500, seed=3), srv._gce(4), 500)
+    before = job.state.occ.clone(), job.state.energy.clone(), job.state.step.clone()
+    tile = autotune_tile(srv.ctx, [job.state], 1, n_steps=64)
+    torch.cuda.synchronize()
+    assert tile in (4, 8, 16)
+    assert torch.equal(before[0], job.state.occ) and torch.equal(before[1], job.state.energy)
+    assert torch.equal(before[2], job.state.step)
+
+
+def test_admission_respects_the_memory_budget(torch_dev):
+    from neuralce2_amd.serve import Server, JobSpec, _CEJob
+    specs = [JobSpec(i, n=4, x=0.5, n_temps=8, n_steps=1000, seed=i) for i in range(4)]
+    budget = 2 * _CEJob.bytes_needed(specs[0]) + 10       # two jobs at a time
+    srv = Server(chunk=1000, mem_budget=budget, autotune=False)
+    res, stats = srv.run(specs)
+    assert stats["n_jobs"] == 4
+    adm = sorted(r.admitted for r in res)
+    fin = sorted(r.finished for r in res)
+    assert adm[2] >= fin[0] - 1e-9                        # the third job waits for a retirement
