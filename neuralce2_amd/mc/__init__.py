@@ -1,0 +1,1 @@
+"""Monte Carlo entry points (the neuralce-mc command line)."""

@@ -167,8 +167,20 @@ def run_pt_mc(model_path, init_cif, n_steps=10000, T_min=100.0, T_max=2000.0, n_
     mc = _prepare(model_path, init_cif, supercell, seed_idx, dtype, seed=seed)
     temps = np.asarray(T_grid, float) if T_grid is not None else np.geomspace(T_min, T_max, n_replicas)
     betas = [1.0 / (KB * t) for t in temps]
+    sig0 = None
+    if init_npz:
+        from ..mc.init_structures import load_init_npz
+        from ..lattice import ATOMIC_NUMBERS
+        nums, _ = load_init_npz(init_npz, mc["mdl"]["primitive"], mc["atoms"], temps)
+        lut = np.full(256, -1, np.int64)
+        for k, sp in enumerate(mc["nn"].geo["species"]):
+            lut[ATOMIC_NUMBERS[sp]] = k
+        sig = lut[nums.astype(np.int64)]
+        if (sig < 0).any():
+            raise ValueError("init_npz holds a species outside the model's")
+        sig0 = sig[:, mc["perm"]].astype(np.int8)
     res = _run_replicas(mc, betas, n_steps, seed, min(chunk, n_steps), log_every, sample_interval, out_dir,
-                        "ce_cnn PT", exchange_every=exchange_every, anchor_every=anchor_every)
+                        "ce_cnn PT", exchange_every=exchange_every, anchor_every=anchor_every, init_npz=sig0)
     res.update(temps=[float(t) for t in temps], exchange_every=int(exchange_every))
     return res
 
