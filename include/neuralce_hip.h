@@ -209,6 +209,13 @@ int nce_nn_full_cache(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_
 int nce_nn_run_chunk(nce_ctx *ctx, const nce_nn_job *jobs, int32_t n_jobs, int64_t n_steps, int32_t precision,
                      void *stream);
 
+/* Energy change (model units, as the running totals) of one swap per replica, the proposals given in
+ * inj_ia / inj_ib (inj_u is not used), evaluated by the local update WITHOUT committing: occupations, caches,
+ * energies and counters are left as they are.  dE: DEVICE array [total replicas of all jobs, in job order]; a
+ * same-site or same-species proposal gives 0.  Replaces local_delta (neuralce/nce/local_update_implicit.py:117-183,
+ * setup()['local_delta'] of neuralce/nce/mc_model.py:221-228); commit is a forced-accept step (u = 0). */
+int nce_nn_delta(nce_ctx *ctx, const nce_nn_job *jobs, int32_t n_jobs, double *dE, int32_t precision, void *stream);
+
 /* Philox4x32-10 of the library (known-answer checks). */
 void nce_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -79,6 +79,7 @@ def _load():
     lib.nce_nn_sizes.argtypes = [_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.nce_nn_full_cache.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p, C.c_int32, _p]
     lib.nce_nn_run_chunk.argtypes = [_p, _p, C.c_int32, C.c_int64, C.c_int32, _p]
+    lib.nce_nn_delta.argtypes = [_p, _p, C.c_int32, _p, C.c_int32, _p]
     lib.nce_philox4x32_10.argtypes = [_p, _p, _p]
     lib.nce_philox4x32_10.restype = None
     return lib
@@ -88,7 +89,7 @@ lib = _load()
 
 EXPORTED = ("nce_last_error", "nce_ctx_create", "nce_ctx_destroy", "nce_ce_model_create", "nce_ce_energy",
             "nce_ce_delta", "nce_pt_run_chunk", "nce_pt_exchange", "nce_philox4x32_10", "nce_nn_model_create",
-            "nce_nn_sizes", "nce_nn_full_cache", "nce_nn_run_chunk")
+            "nce_nn_sizes", "nce_nn_full_cache", "nce_nn_run_chunk", "nce_nn_delta")
 
 
 class NCEError(RuntimeError):
@@ -245,3 +246,8 @@ class Context:
         arr = (NNJob * len(jobs))(*jobs)
         check(lib.nce_nn_run_chunk(self.h, C.addressof(arr), len(jobs), int(n_steps), int(precision),
                                    stream_ptr(stream)))
+
+    def nn_delta(self, jobs, dE, precision=PREC_MIXED, stream=None):
+        """energy change of the injected proposal of every replica, nothing committed (dE: device float64)."""
+        arr = (NNJob * len(jobs))(*jobs)
+        check(lib.nce_nn_delta(self.h, C.addressof(arr), len(jobs), ptr(dE), int(precision), stream_ptr(stream)))

@@ -289,7 +289,8 @@ typedef float nn_floatx4 __attribute__((ext_vector_type(4)));
 template <typename Real, int MODE, bool MFMA>
 __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__restrict__ models,
                                                          const nce_nn_job *__restrict__ jobs,
-                                                         const int2 *__restrict__ items, int64_t n_steps) {
+                                                         const int2 *__restrict__ items, int64_t n_steps,
+                                                         double *__restrict__ eval_dE) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int2 it = items[blockIdx.x];
     const nce_nn_job job = jobs[it.x];
@@ -338,7 +339,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
         }
         const int si = occ[ia], sj = occ[ib];
         if (ia == ib || si == sj) {                      // no-op proposal: dE = 0, accepted (reference semantics)
-            ++nacc;
+            if (eval_dE) {
+                if (tid == 0) eval_dE[blockIdx.x] = 0.0;
+            } else {
+                ++nacc;
+            }
         } else {
             // ball 0: the two layer-0 windows
             for (int k = tid; k < 2 * m.W0; k += NN_BS) {
@@ -568,7 +573,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             const double diff = nn_block_sum<double>(part, red);
             const double dE = (m.readout == 0 ? diff / (double)m.nA : diff);
             const double dE_eV = dE * m.de_scale;
-            const bool accept = (dE_eV <= 0.0) || (u < exp(-beta * dE_eV));
+            // evaluation only (nce_nn_delta): report the change, leave every state buffer untouched
+            const bool accept = eval_dE == nullptr && ((dE_eV <= 0.0) || (u < exp(-beta * dE_eV)));
+            if (eval_dE && tid == 0) eval_dE[blockIdx.x] = dE;
             if (accept) {
                 for (int k = tid; k < counts[0] * C0; k += NN_BS) {
                     const int s = k / C0, c = k - s * C0;
@@ -596,11 +603,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
             if (tid == 0) misc[0] = 0;
         }
-        if (job.traj && ((t + 1) % job.sample_interval) == 0 && tid == 0)
+        if (job.traj && eval_dE == nullptr && ((t + 1) % job.sample_interval) == 0 && tid == 0)
             job.traj[((t + 1) / job.sample_interval - 1) * job.n_replicas + r] = E;
         __syncthreads();
     }
-    if (tid == 0) {
+    if (tid == 0 && eval_dE == nullptr) {
         job.energy[r] = E;
         job.n_accept[r] += nacc;
         job.step[r] = (int64_t)(step0 + (uint64_t)n_steps);
@@ -877,8 +884,24 @@ extern "C" int nce_nn_full_cache(nce_ctx *c, int32_t model, const int8_t *occ, i
     return set_err(NCE_ERR_ARG, "unknown precision");
 }
 
+static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t n_steps, int32_t precision,
+                     void *stream, double *eval_dE);
+
 extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t n_steps,
                                 int32_t precision, void *stream) {
+    return nn_launch(c, jobs, n_jobs, n_steps, precision, stream, nullptr);
+}
+
+extern "C" int nce_nn_delta(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, double *dE, int32_t precision,
+                            void *stream) {
+    if (!dE) return set_err(NCE_ERR_ARG, "NULL dE");
+    for (int j = 0; j < n_jobs && jobs; ++j)
+        if (!jobs[j].inj_ia) return set_err(NCE_ERR_ARG, "nce_nn_delta needs the proposals in inj_ia / inj_ib / inj_u");
+    return nn_launch(c, jobs, n_jobs, 1, precision, stream, dE);
+}
+
+static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t n_steps, int32_t precision,
+                     void *stream, double *eval_dE) {
     if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
     if (n_steps < 0) return set_err(NCE_ERR_ARG, "negative n_steps");
     if (precision != NCE_PREC_FP64 && precision != NCE_PREC_MIXED && precision != NCE_PREC_BF16)
@@ -923,7 +946,8 @@ extern "C" int nce_nn_run_chunk(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jo
     const NNModelDev *dm = (const NNModelDev *)c->d_nn_models;
     auto launch = [&](auto kernel) -> int {
         if (int rc = allow_lds(kernel, lds)) return rc;
-        hipLaunchKernelGGL(kernel, dim3((unsigned)items.size()), dim3(NN_BS), lds, s, dm, d_jobs, d_items, n_steps);
+        hipLaunchKernelGGL(kernel, dim3((unsigned)items.size()), dim3(NN_BS), lds, s, dm, d_jobs, d_items, n_steps,
+                           eval_dE);
         HIP_TRY(hipGetLastError());
         return NCE_OK;
     };

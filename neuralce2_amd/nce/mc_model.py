@@ -92,7 +92,10 @@ def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0):
         return float(E[0].item()) * nn.ys + nn.ym
 
     def full_cache(sig, spin_state=None):
-        return nn.full_cache(nn.occ(sig), precision=prec)
+        """device state of one configuration (occupations, caches, running energy in model units), the `cache` that
+        local_delta and commit take (local_update_implicit.py:89-104)."""
+        from .mcmc import NNJobState
+        return NNJobState(nn, np.asarray(sig), [1.0], precision=prec)
 
     def local_delta_full(sig, flips):
         """energy change (model units) of applying `flips` [[site, species], ...], from two full recomputes."""
@@ -102,8 +105,45 @@ def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0):
         _, E = nn.full_cache(nn.occ(np.stack([s0, s1])), precision=0)
         return float(E[1].item() - E[0].item())
 
+    def local_delta(cache, sig, flips, spin_state=None, sflips=None):
+        """(dE in model units, patch, sig1) of the swap flips = [[i, sigma_j], [j, sigma_i]] by the local update on
+        the GPU, nothing committed (local_update_implicit.py:117-183).  `cache` is a full_cache(sig) state."""
+        import torch
+        if sflips is not None:
+            raise NotImplementedError("spin flips are not part of the canonical swap path")
+        fl = np.asarray(flips, np.int64).reshape(2, 2)
+        i, j = int(fl[0, 0]), int(fl[1, 0])
+        sig1 = np.asarray(sig).copy()
+        sig1[i], sig1[j] = fl[0, 1], fl[1, 1]
+        if not (int(fl[0, 1]) == int(np.asarray(sig)[j]) and int(fl[1, 1]) == int(np.asarray(sig)[i])):
+            raise ValueError("flips must be the swap [[i, sigma[j]], [j, sigma[i]]]")
+        inj = (torch.tensor([[i]], device=nn.device), torch.tensor([[j]], device=nn.device),
+               torch.zeros((1, 1), dtype=torch.float64, device=nn.device))
+        dE = torch.empty(1, dtype=torch.float64, device=nn.device)
+        nn.ctx.nn_delta([cache.job(inj)], dE, prec)
+        return float(dE.item()), (i, j), sig1
+
+    def commit(cache, patch, accept):
+        """apply the swap `patch` to the caches, occupations and running energy when accept (a forced-accept step
+        of the same proposal; local_update_implicit.py:185-194)."""
+        import torch
+        if not accept:
+            return cache
+        i, j = patch
+        inj = (torch.tensor([[i]], device=nn.device), torch.tensor([[j]], device=nn.device),
+               torch.zeros((1, 1), dtype=torch.float64, device=nn.device))
+        cache.run(1, inj)
+        return cache
+
+    def propose(key):
+        """(i, j): two sites uniform over the cell from the counter-based stream of `key` (an int)."""
+        from .._lib import philox4x32_10
+        x = [int(v) for v in philox4x32_10([int(key) & 0xffffffff, (int(key) >> 32) & 0xffffffff, 0, 0], [0x5EED, 0])]
+        return (x[0] | (x[1] << 32)) % nn.N, (x[2] | (x[3] << 32)) % nn.N
+
     return dict(cs=cs, P=P, mu=mu, sd=sd, depth=nn.depth, ys=nn.ys, ym=nn.ym, n_atoms=nn.N, n_active=nn.nA,
                 sig0=geo["sig"], energy=energy, full_cache=full_cache, local_delta_full=local_delta_full,
+                local_delta=local_delta, commit=commit, propose=propose,
                 to_eV_per_atom=lambda E_std: E_std * nn.ys + nn.ym, dE_eV_total=lambda dE: dE * nn.ys * nn.N,
                 implicit=True, geo=geo, perm=geo["perm"], inv_perm=geo["inv_perm"], nn=nn, dtype=dtype,
                 precision=prec, cache_mode="h0")

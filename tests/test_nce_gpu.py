@@ -211,3 +211,34 @@ def test_bf16_mfma_energy_change(torch_dev, ptni_nce, kind):
     scale = np.max(np.abs(ref)) + 1e-5
     assert np.max(np.abs(E_run - E_full)) <= steps * 3e-2 * scale, (np.abs(E_run - E_full).max(), scale)
     print(kind, "max |dE err| / |dE|", float(np.max(err / (np.abs(ref) + 1e-12))), "chain drift", float(np.abs(E_run - E_full).max()))
+
+
+def test_setup_local_delta_and_commit(torch_dev, ptni_nce):
+    """setup(mdl, atoms): local_delta gives the oracle's energy difference without changing the state; commit applies
+    the swap so that the carried caches equal a rebuild (the reference's setup()['local_delta'] / ['commit'])."""
+    from neuralce2_amd.nce.mc_model import setup
+    from oracle.nce_oracle import Oracle
+    mdl, cs, P, mu, sd = ptni_nce
+    at = _cell("3", 5)
+    mc = setup(mdl, at, dtype="float64")
+    o = Oracle(cs, at, mdl["shell_edges"], P, mu, sd)
+    perm = mc["perm"]
+    sig = mc["sig0"].astype(np.int8).copy()
+    cache = mc["full_cache"](sig)
+    rng = np.random.default_rng(9)
+    for _ in range(6):
+        i, j = mc["propose"](int(rng.integers(1 << 40)))
+        if sig[i] == sig[j]:
+            j = int(np.flatnonzero(sig != sig[i])[0])
+        before = cache.cache.clone()
+        dE, patch, sig1 = mc["local_delta"](cache, sig, [[i, sig[j]], [j, sig[i]]])
+        assert cache.cache.equal(before)
+        s_o = np.empty_like(sig); s_o[perm] = sig
+        s1_o = np.empty_like(sig1); s1_o[perm] = sig1
+        assert abs(dE - (o.energy(s1_o) - o.energy(s_o))) < 1e-10
+        cache = mc["commit"](cache, patch, True)
+        sig = sig1
+        assert np.array_equal(cache.occ.cpu().numpy()[0], sig)
+    carried = cache.cache.clone()
+    cache.anchor()
+    assert (carried - cache.cache).abs().max().item() < 1e-9
