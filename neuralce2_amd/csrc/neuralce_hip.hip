@@ -982,9 +982,13 @@ __global__ void ce_energy_finish_kernel(const ModelDev *__restrict__ models, int
 }
 
 // ───────────────────────────────────────────── exchange kernel ────────────────────────────────────
-__global__ void pt_exchange_kernel(const nce_pt_exchange_job *__restrict__ jobs) {
+// One workgroup per job.  The pairs of one parity are disjoint, so each lane takes pairs p = lane, lane + 256, ...
+// with its own Philox uniform (counter (ex_index, p)); the new slot maps are then rewritten in parallel.
+constexpr int EXCH_BS = 256;
+
+__global__ __launch_bounds__(EXCH_BS) void pt_exchange_kernel(const nce_pt_exchange_job *__restrict__ jobs) {
+    __shared__ int s_ns;
     const nce_pt_exchange_job j = jobs[blockIdx.x];
-    if (threadIdx.x != 0) return;
     const int R = j.n_replicas;
     const uint32_t k0 = (uint32_t)j.seed, k1 = (uint32_t)(j.seed >> 32);
     int parity;
@@ -995,8 +999,10 @@ __global__ void pt_exchange_kernel(const nce_pt_exchange_job *__restrict__ jobs)
         philox4x32_10(z, k0, k1);
         parity = (int)(z[0] & 1u);
     }
-    int64_t ns = 0;
-    for (int p = 0; 2 * p <= R - 2; ++p) {
+    if (threadIdx.x == 0) s_ns = 0;
+    __syncthreads();
+    int ns = 0;
+    for (int p = threadIdx.x; 2 * p <= R - 2; p += EXCH_BS) {
         const int a = 2 * p + parity, b = a + 1;
         if (b >= R) continue;
         if (j.ladder_id && j.ladder_id[a] != j.ladder_id[b]) continue;
@@ -1008,21 +1014,23 @@ __global__ void pt_exchange_kernel(const nce_pt_exchange_job *__restrict__ jobs)
             philox4x32_10(w, k0, k1);
             u = ((double)((((uint64_t)w[1] << 32) | w[0]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
         }
-        const double la = (j.beta_slot[b] - j.beta_slot[a]) * (j.energy[j.rep_of_slot[b]] - j.energy[j.rep_of_slot[a]]);
+        const int ra = j.rep_of_slot[a], rb = j.rep_of_slot[b];
+        const double la = (j.beta_slot[b] - j.beta_slot[a]) * (j.energy[rb] - j.energy[ra]);
         if (log(u) < la) {
-            const int t = j.rep_of_slot[a];
-            j.rep_of_slot[a] = j.rep_of_slot[b];
-            j.rep_of_slot[b] = t;
+            j.rep_of_slot[a] = rb;
+            j.rep_of_slot[b] = ra;
             ++ns;
         }
     }
-    for (int t = 0; t < R; ++t) {
+    if (ns) atomicAdd(&s_ns, ns);
+    __syncthreads();
+    for (int t = threadIdx.x; t < R; t += EXCH_BS) {
         const int rep = j.rep_of_slot[t];
         j.beta_rep[rep] = j.beta_slot[t];
         j.slot_of_rep[rep] = t;
         if (j.traj_last) j.traj_last[t] = j.energy[rep];
     }
-    j.n_swaps[0] += ns;
+    if (threadIdx.x == 0) j.n_swaps[0] += s_ns;
 }
 
 // ───────────────────────────────────────────── host side ──────────────────────────────────────────
@@ -1509,7 +1517,7 @@ extern "C" int nce_pt_exchange(nce_ctx *c, const nce_pt_exchange_job *jobs, int3
     void *dev = nullptr;
     int rc = stage_upload(c, jobs, sizeof(nce_pt_exchange_job) * n_jobs, s, &dev);
     if (rc) return rc;
-    hipLaunchKernelGGL(pt_exchange_kernel, dim3(n_jobs), dim3(64), 0, s, (const nce_pt_exchange_job *)dev);
+    hipLaunchKernelGGL(pt_exchange_kernel, dim3(n_jobs), dim3(EXCH_BS), 0, s, (const nce_pt_exchange_job *)dev);
     HIP_TRY(hipGetLastError());
     return NCE_OK;
 }
