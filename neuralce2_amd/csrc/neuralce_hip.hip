@@ -662,20 +662,23 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             using CT = typename std::conditional<sizeof(Real) == 8, uint64_t, uint32_t>::type;
             Real accA = (Real)0, accB = (Real)0;
             const uint32_t a32 = (uint32_t)WA[0], b32 = (uint32_t)WB[0];
+            // uniform trip counts: the loop control stays on the scalar unit
+            const FsLo<Real> *plo = s_lo + lane;
 #pragma unroll 4
-            for (int i = lane; i < 64 * nlo; i += 64) {
-                const FsLo<Real> e = s_lo[i];
+            for (int it = 0; it < nlo; ++it) {
+                const FsLo<Real> e = plo[64 * it];
                 accA += fs_term((CT)e.c, (uint32_t)__builtin_popcount(a32 & e.m));
                 accB += fs_term((CT)e.c, (uint32_t)__builtin_popcount(b32 & e.m));
             }
+            const FsFull *pfu = s_full + lane;
 #pragma unroll 4
-            for (int i = lane; i < 64 * nfull; i += 64) {
-                const FsFull e = s_full[i];
+            for (int it = 0; it < nfull; ++it) {
+                const FsFull e = pfu[64 * it];
                 accA += fs_term((CT)e.c, (uint32_t)__popcll(WA[0] & e.m));
                 accB += fs_term((CT)e.c, (uint32_t)__popcll(WB[0] & e.m));
             }
-            for (int i = lane; i < 64 * ntail; i += 64) {
-                const FsTail e = s_tail[i];
+            for (int it = 0; it < ntail; ++it) {
+                const FsTail e = s_tail[64 * it + lane];
                 const unsigned long long a1 = e.word == 1 ? WA[1] : WA[2], b1 = e.word == 1 ? WB[1] : WB[2];
                 accA += fs_term((CT)e.c, (uint32_t)__popcll((WA[0] & e.m0) ^ (a1 & e.mk)));
                 accB += fs_term((CT)e.c, (uint32_t)__popcll((WB[0] & e.m0) ^ (b1 & e.mk)));
