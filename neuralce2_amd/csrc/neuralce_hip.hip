@@ -547,7 +547,14 @@ struct SpecRec {
 
 template <typename Real>
 struct SpecLds {
-    size_t slot, coef, val, prop, rec, commit, fs, total;
+    size_t slot, coef, val, prop, rec, commit, fs, pp, meta, total;
+};
+
+constexpr int SPEC_KP = 64;            // proposals scanned per group (one lane of wave 0 each)
+
+struct SpecProp {
+    int ia, ib;
+    double u;
 };
 
 template <typename Real>
@@ -561,6 +568,8 @@ __host__ __device__ inline SpecLds<Real> spec_lds_layout(int J, int Wcap, int wa
     L.rec = o;    o = align16(o + (size_t)waves * sizeof(SpecRec));
     L.commit = o; o = align16(o + (size_t)waves * sizeof(int4));
     L.fs = o;     o = align16(o + fs_bytes<Real>(nlo, nfull, ntail));
+    L.pp = o;     o = align16(o + (size_t)SPEC_KP * sizeof(SpecProp));
+    L.meta = o;   o = align16(o + 2 * sizeof(unsigned long long));
     L.total = o;
     return L;
 }
@@ -588,6 +597,8 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     int2 *s_prop = (int2 *)(smem + L.prop);
     SpecRec *s_rec = (SpecRec *)(smem + L.rec);
     int4 *s_commit = (int4 *)(smem + L.commit);
+    SpecProp *s_pp = (SpecProp *)(smem + L.pp);
+    unsigned long long *s_meta = (unsigned long long *)(smem + L.meta);
     const FsLo<Real> *s_lo = (const FsLo<Real> *)(smem + L.fs);
     const FsFull *s_full = (const FsFull *)(s_lo + 64 * nlo);
     const FsTail *s_tail = (const FsTail *)(s_full + 64 * nfull);
@@ -716,18 +727,64 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         else return (double)((float)sum * (float)factor);
     };
 
-    for (int64_t t0 = 0; t0 < n_steps; t0 += W) {
-        const int64_t t = t0 + w;
-        const bool active = t < n_steps;
-        // 1. proposal of this wave's step and its window loads
-        int ia, ib;
-        double u;
-        if constexpr (INJ) {
-            const int64_t tc = active ? t : n_steps - 1;
-            ia = (int)inj_ia[inj_base + tc]; ib = (int)inj_ib[inj_base + tc]; u = inj_u[inj_base + tc];
-        } else {
-            const Proposal p = draw_proposal_1b(seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)n_cells);
-            ia = (int)p.ia; ib = (int)p.ib; u = p.u;
+    int64_t nstep = 0;
+    for (int64_t t0 = 0; t0 < n_steps; t0 += nstep) {
+        // 0. proposals of the next SPEC_KP steps (wave 0, one lane each) and the species of both sites.  A swap of
+        // two equal species changes nothing and is rejected, so the group is the run of steps holding the next W
+        // different-species swaps (one wave each).  It ends before an equal-species step whose sites an earlier
+        // swap of the group may change: every equal-species step inside the group therefore stays one.
+        if (w == 0) {
+            const int64_t t = t0 + lane;
+            const bool act = t < n_steps;
+            int pia, pib;
+            double pu;
+            if constexpr (INJ) {
+                const int64_t tc = act ? t : n_steps - 1;
+                pia = (int)inj_ia[inj_base + tc]; pib = (int)inj_ib[inj_base + tc]; pu = inj_u[inj_base + tc];
+            } else {
+                const Proposal p = draw_proposal_1b(seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)n_cells);
+                pia = (int)p.ia; pib = (int)p.ib; pu = p.u;
+            }
+            const int sa0 = act ? (int)occ[pia] : 0, sb0 = act ? (int)occ[pib] : 0;
+            const bool v = act && sa0 != sb0;
+            s_pp[lane] = SpecProp{pia, pib, pu};
+            const uint64_t vm = __ballot(v);
+            const int n_act = (int)((n_steps - t0) < (int64_t)SPEC_KP ? (n_steps - t0) : (int64_t)SPEC_KP);
+            int end = n_act;
+            if (__popcll(vm) >= W) {                   // after the W-th different-species swap
+                uint64_t m = vm;
+                for (int k = 1; k < W; ++k) m &= m - 1;
+                end = min(end, __builtin_ctzll(m) + 1);
+            }
+            // an equal-species step whose sites meet those of an earlier different-species step ends the group
+            bool touched = false;
+            uint64_t m = vm & (end >= 64 ? ~0ull : ((1ull << end) - 1ull));
+            while (m) {
+                const int j = __builtin_ctzll(m);
+                m &= m - 1;
+                const int ja = __builtin_amdgcn_readlane(pia, j), jb = __builtin_amdgcn_readlane(pib, j);
+                touched |= lane > j && (pia == ja || pia == jb || pib == ja || pib == jb);
+            }
+            const uint64_t tm = __ballot(act && !v && touched && lane < end);
+            if (tm) end = min(end, __builtin_ctzll(tm));
+            if (lane == 0) {
+                s_meta[0] = vm & (end >= 64 ? ~0ull : ((1ull << end) - 1ull));
+                s_meta[1] = (unsigned long long)end;
+            }
+        }
+        __syncthreads();
+        const uint64_t gvm = s_meta[0];
+        nstep = (int64_t)s_meta[1];
+        const int nW = __popcll(gvm);                // different-species swaps of the group, one per wave
+        const bool active = w < nW;
+        // 1. this wave's swap and its window loads
+        int ia = 0, ib = 0;
+        double u = 0.0;
+        if (active) {
+            uint64_t m = gvm;
+            for (int k = 0; k < w; ++k) m &= m - 1;
+            const SpecProp pp = s_pp[__builtin_ctzll(m)];
+            ia = pp.ia; ib = pp.ib; u = pp.u;
         }
         ia = __builtin_amdgcn_readfirstlane(ia);
         ib = __builtin_amdgcn_readfirstlane(ib);
@@ -786,8 +843,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         // then every step whose windows hold a site changed by an accepted earlier step of the group is
         // re-evaluated on the patched species by its own wave, in step order (all waves follow the same uniform
         // sequence of barriers).
-        const int nW = (int)((n_steps - t0) < (int64_t)W ? (n_steps - t0) : (int64_t)W);
-        SpecRec ri = s_rec[lane < nW ? lane : nW - 1];
+        SpecRec ri = s_rec[lane < nW ? lane : 0];
         auto accepts = [&](const SpecRec &q) __attribute__((always_inline)) -> bool {
             if (!q.valid) return false;
             if constexpr (sizeof(Real) == 8) return q.lu < -beta * q.dE;
@@ -860,11 +916,15 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             nacc += __popcll(accm);
             const int lo32 = (int)(uint32_t)__double_as_longlong(ri.dE);
             const int hi32 = (int)(__double_as_longlong(ri.dE) >> 32);
-            for (int i = 0; i < nW; ++i) {
-                if ((accm >> i) & 1ull) {
-                    const uint32_t l = (uint32_t)__builtin_amdgcn_readlane(lo32, i);
-                    const uint32_t h = (uint32_t)__builtin_amdgcn_readlane(hi32, i);
-                    E += __longlong_as_double((long long)(((uint64_t)h << 32) | l));
+            int k = 0;                                 // rank of the next different-species swap
+            for (int i = 0; i < (int)nstep; ++i) {
+                if ((gvm >> i) & 1ull) {
+                    if ((accm >> k) & 1ull) {
+                        const uint32_t l = (uint32_t)__builtin_amdgcn_readlane(lo32, k);
+                        const uint32_t h = (uint32_t)__builtin_amdgcn_readlane(hi32, k);
+                        E += __longlong_as_double((long long)(((uint64_t)h << 32) | l));
+                    }
+                    ++k;
                 }
                 emin = fmin(emin, E);
                 if (accumulate) {
