@@ -914,30 +914,36 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                     }
             }
             nacc += __popcll(accm);
-            const int lo32 = (int)(uint32_t)__double_as_longlong(ri.dE);
-            const int hi32 = (int)(__double_as_longlong(ri.dE) >> 32);
-            int k = 0;                                 // rank of the next different-species swap
-            for (int i = 0; i < (int)nstep; ++i) {
-                if ((gvm >> i) & 1ull) {
-                    if ((accm >> k) & 1ull) {
-                        const uint32_t l = (uint32_t)__builtin_amdgcn_readlane(lo32, k);
-                        const uint32_t h = (uint32_t)__builtin_amdgcn_readlane(hi32, k);
-                        E += __longlong_as_double((long long)(((uint64_t)h << 32) | l));
-                    }
-                    ++k;
-                }
-                emin = fmin(emin, E);
-                if (accumulate) {
-                    const double x = E - eref;
-                    s1 += x;
-                    s2 += x * x;
-                }
-                if (traj && t0 + i + 1 == next_sample) {
-                    if (lane == 0) traj[n_sampled * n_rep + slot_r] = E;
-                    next_sample += sample_interval;
-                    ++n_sampled;
-                }
+            // lane i = step i of the group: its energy change (the k-th swap's, if accepted), the running total
+            // after it by an inclusive prefix sum, then minimum, moment sums and trajectory samples from the lanes
+            const int kl = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(gvm >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)gvm, 0u));
+            const bool in = lane < (int)nstep;
+            const bool acc_i = in && ((gvm >> lane) & 1ull) && ((accm >> kl) & 1ull);
+            const double dk = __shfl(ri.dE, kl < 64 ? kl : 0, 64);
+            double P = acc_i ? dk : 0.0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double v = __shfl_up(P, (unsigned)o, 64);
+                if (lane >= o) P += v;
             }
+            const double Ei = E + P;
+            double mn = in ? Ei : E;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) mn = fmin(mn, __shfl_xor(mn, o, 64));
+            emin = fmin(emin, mn);
+            if (accumulate) {
+                const double x = in ? Ei - eref : 0.0;
+                s1 += wave_sum(x);
+                s2 += wave_sum(x * x);
+            }
+            while (traj && next_sample <= t0 + nstep) {
+                const double Es = __shfl(Ei, (int)(next_sample - t0 - 1), 64);
+                if (lane == 0) traj[n_sampled * n_rep + slot_r] = Es;
+                next_sample += sample_interval;
+                ++n_sampled;
+            }
+            E = __shfl(Ei, (int)nstep - 1, 64);
         }
         __syncthreads();
     }

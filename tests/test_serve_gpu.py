@@ -1,7 +1,9 @@
 """Served PT jobs (continuous batching of many jobs per launch): each job's result is the same alone and in the mix.
 
-Cluster-expansion jobs: bit-identical final occupations and per-temperature <E>, C_v (Philox streams keyed by job
-seed, replica and step; the speculative kernel's decisions are the sequential chain's whatever the tile).  NeuralCE
+Cluster-expansion jobs: bit-identical final occupations (Philox streams keyed by job seed, replica and step; the
+speculative kernel's decisions are the sequential chain's whatever the tile), per-temperature <E> within 1e-12 and
+C_v within 1e-6 relative (the FP64 moment sums of a group are taken by wave reductions, so their association
+follows the grouping).  NeuralCE
 jobs: <E>(T) within 1e-6 eV/atom (their energy sums are FP32 with a per-step order that depends on LDS atomics)."""
 import os
 
@@ -52,7 +54,7 @@ def test_results_same_alone_and_in_the_mix(torch_dev):
         assert r.job_id == a.job_id == s.job_id
         if s.kind == "ce":
             assert np.array_equal(r.occ, a.occ), s.job_id
-            assert np.array_equal(r.E_mean, a.E_mean) and np.array_equal(r.C_v, a.C_v), s.job_id
+            assert np.allclose(r.E_mean, a.E_mean, rtol=1e-12, atol=0) and np.allclose(r.C_v, a.C_v, rtol=1e-6, atol=0), s.job_id
         else:
             assert np.max(np.abs(r.E_mean - a.E_mean)) < 1e-6, (r.E_mean, a.E_mean)
         assert np.all(np.isfinite(r.C_v)) and np.all(r.C_v >= 0)
