@@ -205,6 +205,14 @@ __device__ inline double readlane_f64(double v, int lane) {
     return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
 }
 
+// workgroup barrier that orders LDS only: global loads stay in flight across it (a plain __syncthreads waits for
+// every outstanding load and store of the wave first)
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // wave sum, result uniform in every lane; fixed order (deterministic)
 __device__ inline double wave_sum(double v) {
     v += dpp_f64<0xB1>(v);
@@ -634,7 +642,6 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     const uint32_t m12 = mg.div12_m, s12 = mg.div12_s, m2 = mg.div2_m, sh2 = mg.div2_s;
     const uint32_t d12 = (uint32_t)(d1 * d2);
     const double factor = mg.factor;
-    __syncthreads();
 
     NCE_GLOBAL int8_t *occ = gptr(job.occ) + (int64_t)r * job.n_sites;
     const NCE_GLOBAL int64_t *inj_ia = gptr(job.inj_ia), *inj_ib = gptr(job.inj_ib);
@@ -727,54 +734,65 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         else return (double)((float)sum * (float)factor);
     };
 
-    int64_t nstep = 0;
-    for (int64_t t0 = 0; t0 < n_steps; t0 += nstep) {
-        // 0. proposals of the next SPEC_KP steps (wave 0, one lane each) and the species of both sites.  A swap of
-        // two equal species changes nothing and is rejected, so the group is the run of steps holding the next W
-        // different-species swaps (one wave each).  It ends before an equal-species step whose sites an earlier
-        // swap of the group may change: every equal-species step inside the group therefore stays one.
-        if (w == 0) {
-            const int64_t t = t0 + lane;
-            const bool act = t < n_steps;
-            int pia, pib;
-            double pu;
-            if constexpr (INJ) {
-                const int64_t tc = act ? t : n_steps - 1;
-                pia = (int)inj_ia[inj_base + tc]; pib = (int)inj_ib[inj_base + tc]; pu = inj_u[inj_base + tc];
-            } else {
-                const Proposal p = draw_proposal_1b(seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)n_cells);
-                pia = (int)p.ia; pib = (int)p.ib; pu = p.u;
-            }
-            const int sa0 = act ? (int)occ[pia] : 0, sb0 = act ? (int)occ[pib] : 0;
-            const bool v = act && sa0 != sb0;
-            s_pp[lane] = SpecProp{pia, pib, pu};
-            const uint64_t vm = __ballot(v);
-            const int n_act = (int)((n_steps - t0) < (int64_t)SPEC_KP ? (n_steps - t0) : (int64_t)SPEC_KP);
-            int end = n_act;
-            if (__popcll(vm) >= W) {                   // after the W-th different-species swap
-                uint64_t m = vm;
-                for (int k = 1; k < W; ++k) m &= m - 1;
-                end = min(end, __builtin_ctzll(m) + 1);
-            }
-            // an equal-species step whose sites meet those of an earlier different-species step ends the group
-            bool touched = false;
-            uint64_t m = vm & (end >= 64 ? ~0ull : ((1ull << end) - 1ull));
-            while (m) {
-                const int j = __builtin_ctzll(m);
-                m &= m - 1;
-                const int ja = __builtin_amdgcn_readlane(pia, j), jb = __builtin_amdgcn_readlane(pib, j);
-                touched |= lane > j && (pia == ja || pia == jb || pib == ja || pib == jb);
-            }
-            const uint64_t tm = __ballot(act && !v && touched && lane < end);
-            if (tm) end = min(end, __builtin_ctzll(tm));
-            if (lane == 0) {
-                s_meta[0] = vm & (end >= 64 ? ~0ull : ((1ull << end) - 1ull));
-                s_meta[1] = (unsigned long long)end;
-            }
+    // Group formation (wave 0, one lane per step).  Lanes 0..63 hold the proposals of steps tb..tb+63 and the
+    // species at both sites.  A swap of two equal species changes nothing and is rejected, so the group is the run
+    // of steps holding the next W different-species swaps (one wave each).  It ends before an equal-species step
+    // whose sites an earlier swap of the group may change: every equal-species step inside the group therefore
+    // stays one.  The next batch is drawn and its species loaded while the current group is evaluated; the
+    // species are then patched with the group's commits, in step order, before the next group is formed.
+    int npia = 0, npib = 0, nsa = 0, nsb = 0;
+    double npu = 0.0;
+    auto draw = [&](int64_t tb) __attribute__((always_inline)) {
+        const int64_t t = tb + lane;
+        const bool act = t < n_steps;
+        if constexpr (INJ) {
+            const int64_t tc = act ? t : n_steps - 1;
+            npia = (int)inj_ia[inj_base + tc]; npib = (int)inj_ib[inj_base + tc]; npu = inj_u[inj_base + tc];
+        } else {
+            const Proposal p = draw_proposal_1b(seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)n_cells);
+            npia = (int)p.ia; npib = (int)p.ib; npu = p.u;
         }
-        __syncthreads();
+        nsa = act ? (int)occ[npia] : 0;
+        nsb = act ? (int)occ[npib] : 0;
+    };
+    auto form_group = [&](int64_t tb) __attribute__((always_inline)) {
+        const bool act = tb + lane < n_steps;
+        const bool v = act && nsa != nsb;
+        s_pp[lane] = SpecProp{npia, npib, npu};
+        const uint64_t vm = __ballot(v);
+        const int n_act = (int)((n_steps - tb) < (int64_t)SPEC_KP ? (n_steps - tb) : (int64_t)SPEC_KP);
+        int end = n_act;
+        if (__popcll(vm) >= W) {                       // after the W-th different-species swap
+            uint64_t m = vm;
+            for (int k = 1; k < W; ++k) m &= m - 1;
+            end = min(end, __builtin_ctzll(m) + 1);
+        }
+        // an equal-species step whose sites meet those of an earlier different-species step ends the group
+        bool touched = false;
+        uint64_t m = vm & (end >= 64 ? ~0ull : ((1ull << end) - 1ull));
+        while (m) {
+            const int j = __builtin_ctzll(m);
+            m &= m - 1;
+            const int ja = __builtin_amdgcn_readlane(npia, j), jb = __builtin_amdgcn_readlane(npib, j);
+            touched |= lane > j && (npia == ja || npia == jb || npib == ja || npib == jb);
+        }
+        const uint64_t tm = __ballot(act && !v && touched && lane < end);
+        if (tm) end = min(end, __builtin_ctzll(tm));
+        if (lane == 0) {
+            s_meta[0] = vm & (end >= 64 ? ~0ull : ((1ull << end) - 1ull));
+            s_meta[1] = (unsigned long long)end;
+        }
+    };
+    if (w == 0) {
+        draw(0);
+        form_group(0);
+    }
+    __syncthreads();
+
+    for (int64_t t0 = 0; t0 < n_steps;) {
         const uint64_t gvm = s_meta[0];
-        nstep = (int64_t)s_meta[1];
+        const int64_t nstep = (int64_t)s_meta[1];
+        const int64_t t1 = t0 + nstep;                 // first step of the next group
         const int nW = __popcll(gvm);                // different-species swaps of the group, one per wave
         const bool active = w < nW;
         // 1. this wave's swap and its window loads
@@ -807,8 +825,9 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             pA[j] = wok[j] ? (int)occ[sA[j]] : 1;
             pB[j] = wok[j] ? (int)occ[sB[j]] : 1;
         }
+        if (w == 0 && t1 < n_steps) draw(t1);
         if (lane == 0) s_prop[w] = make_int2(ia, ib);
-        __syncthreads();
+        lds_barrier();
         // 2. conflicts with the earlier steps of the group, centres, near flag, energy change
         int conflict = 0;
         for (int q = 0; q < w; ++q) {
@@ -838,7 +857,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             rec.dE = dE; rec.lu = lu; rec.sa = sa; rec.sb = sb; rec.valid = valid; rec.conflict = conflict;
             s_rec[w] = rec;
         }
-        __syncthreads();
+        lds_barrier();
         // 3. decisions.  Lane i of every wave holds step i's record; the tentative decisions are taken in parallel,
         // then every step whose windows hold a site changed by an accepted earlier step of the group is
         // re-evaluated on the patched species by its own wave, in step order (all waves follow the same uniform
@@ -862,7 +881,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                         s_commit[n++] = make_int4(pi.x, pi.y, q.sb, q.sa);
                     }
             }
-            __syncthreads();
+            lds_barrier();
             if (w == f) {
                 const int n_commit = __popcll(accm & ((1ull << f) - 1ull));
                 int qA[NS], qB[NS];
@@ -888,19 +907,20 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                     s_rec[f] = r2;
                 }
             }
-            __syncthreads();
+            lds_barrier();
             if (lane == f) ri = s_rec[f];
             const uint64_t bit = 1ull << f;
             accm = (accm & ~bit) | (__ballot(lane == f && accepts(ri)) & bit);
             need = __ballot(lane > f && lane < nW && (ri.conflict & (int)accm) != 0);
         }
         // 4. wave 0: occupation writes of the accepted steps (in parallel unless two of them share a site, which
-        // only happens through a re-evaluated step: then in step order), running total, statistics, trajectory
+        // only happens through a re-evaluated step: then in step order), then the next group.  The last wave
+        // (every wave holds the final records and decisions): running total, statistics, trajectory.
         if (w == 0) {
             const bool mine = (accm >> lane) & 1ull;
+            const int2 pl = s_prop[lane < nW ? lane : 0];
             if (__ballot(mine && (ri.conflict & (int)accm) != 0) == 0ull) {
                 if (mine) {
-                    const int2 pl = s_prop[lane];
                     occ[pl.x] = (int8_t)ri.sb;
                     occ[pl.y] = (int8_t)ri.sa;
                 }
@@ -913,6 +933,20 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                         occ[pi.y] = (int8_t)q.sa;
                     }
             }
+            if (t1 < n_steps) {
+                uint64_t m = accm;
+                while (m) {
+                    const int k = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int kx = __builtin_amdgcn_readlane(pl.x, k), ky = __builtin_amdgcn_readlane(pl.y, k);
+                    const int ka = __builtin_amdgcn_readlane(ri.sa, k), kb = __builtin_amdgcn_readlane(ri.sb, k);
+                    if (npia == kx) nsa = kb; else if (npia == ky) nsa = ka;
+                    if (npib == kx) nsb = kb; else if (npib == ky) nsb = ka;
+                }
+                form_group(t1);
+            }
+        }
+        if (w == W - 1) {
             nacc += __popcll(accm);
             // lane i = step i of the group: its energy change (the k-th swap's, if accepted), the running total
             // after it by an inclusive prefix sum, then minimum, moment sums and trajectory samples from the lanes
@@ -945,9 +979,10 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             }
             E = __shfl(Ei, (int)nstep - 1, 64);
         }
-        __syncthreads();
+        __syncthreads();                               // occupation writes and the next group
+        t0 = t1;
     }
-    if (tid == 0) {
+    if (w == W - 1 && lane == 0) {
         job.energy[r] = E;
         job.e_min[r] = emin;
         job.n_accept[r] += nacc;
