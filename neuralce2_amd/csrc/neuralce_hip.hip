@@ -755,26 +755,29 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         nsa = act ? (int)occ[npia] : 0;
         nsb = act ? (int)occ[npib] : 0;
     };
+    // s_pp[k] = the k-th different-species swap of the batch (k < W), the proposal of wave k
     auto form_group = [&](int64_t tb) __attribute__((always_inline)) {
         const bool act = tb + lane < n_steps;
         const bool v = act && nsa != nsb;
-        s_pp[lane] = SpecProp{npia, npib, npu};
         const uint64_t vm = __ballot(v);
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(vm >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)vm, 0u));
+        if (v && rank < W) s_pp[rank] = SpecProp{npia, npib, npu};
         const int n_act = (int)((n_steps - tb) < (int64_t)SPEC_KP ? (n_steps - tb) : (int64_t)SPEC_KP);
         int end = n_act;
-        if (__popcll(vm) >= W) {                       // after the W-th different-species swap
-            uint64_t m = vm;
-            for (int k = 1; k < W; ++k) m &= m - 1;
-            end = min(end, __builtin_ctzll(m) + 1);
-        }
+        const uint64_t wth = __ballot(v && rank == W - 1);
+        if (wth) end = min(end, __builtin_ctzll(wth) + 1);   // after the W-th different-species swap
+        const int nv = min(__popcll(vm), (int)W);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // an equal-species step whose sites meet those of an earlier different-species step ends the group
+        // (entries 0 .. rank-1 are the different-species steps before this lane's)
         bool touched = false;
-        uint64_t m = vm & (end >= 64 ? ~0ull : ((1ull << end) - 1ull));
-        while (m) {
-            const int j = __builtin_ctzll(m);
-            m &= m - 1;
-            const int ja = __builtin_amdgcn_readlane(npia, j), jb = __builtin_amdgcn_readlane(npib, j);
-            touched |= lane > j && (npia == ja || npia == jb || npib == ja || npib == jb);
+#pragma unroll 4
+        for (int k = 0; k < nv; ++k) {
+            const SpecProp q = s_pp[k];
+            touched |= (k < rank) & ((npia == q.ia) | (npia == q.ib) | (npib == q.ia) | (npib == q.ib));
         }
         const uint64_t tm = __ballot(act && !v && touched && lane < end);
         if (tm) end = min(end, __builtin_ctzll(tm));
@@ -799,9 +802,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         int ia = 0, ib = 0;
         double u = 0.0;
         if (active) {
-            uint64_t m = gvm;
-            for (int k = 0; k < w; ++k) m &= m - 1;
-            const SpecProp pp = s_pp[__builtin_ctzll(m)];
+            const SpecProp pp = s_pp[w];
             ia = pp.ia; ib = pp.ib; u = pp.u;
         }
         ia = __builtin_amdgcn_readfirstlane(ia);
@@ -934,14 +935,19 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                     }
             }
             if (t1 < n_steps) {
-                uint64_t m = accm;
-                while (m) {
-                    const int k = __builtin_ctzll(m);
-                    m &= m - 1;
-                    const int kx = __builtin_amdgcn_readlane(pl.x, k), ky = __builtin_amdgcn_readlane(pl.y, k);
-                    const int ka = __builtin_amdgcn_readlane(ri.sa, k), kb = __builtin_amdgcn_readlane(ri.sb, k);
-                    if (npia == kx) nsa = kb; else if (npia == ky) nsa = ka;
-                    if (npib == kx) nsb = kb; else if (npib == ky) nsb = ka;
+                // the group's commits in step order, then every lane of the next batch patches its species
+                const int crank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(accm >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)accm, 0u));
+                if (mine) s_commit[crank] = make_int4(pl.x, pl.y, ri.sb, ri.sa);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int nc = __popcll(accm);
+#pragma unroll 4
+                for (int c = 0; c < nc; ++c) {
+                    const int4 e = s_commit[c];
+                    nsa = npia == e.x ? e.z : (npia == e.y ? e.w : nsa);
+                    nsb = npib == e.x ? e.z : (npib == e.y ? e.w : nsb);
                 }
                 form_group(t1);
             }
