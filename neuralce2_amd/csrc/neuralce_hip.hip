@@ -537,6 +537,8 @@ __host__ __device__ inline size_t fs_bytes(int nlo, int nfull, int ntail) {
 }
 
 // |c| (sign bit clear) with the low bit of n as its sign
+// (a & m) ^ t in one v_bitop3 (truth table over S0 = 0xF0, S1 = 0xCC, S2 = 0xAA)
+__device__ inline uint32_t and_xor(uint32_t a, uint32_t m, uint32_t t) { return __builtin_amdgcn_bitop3_b32(a, m, t, 0x6A); }
 __device__ inline float fs_term(uint32_t c, uint32_t n) { return __int_as_float((int)((n << 31) | c)); }
 __device__ inline double fs_term(uint64_t c, uint32_t n) {
     return __longlong_as_double((long long)(((uint64_t)(n << 31) << 32) | c));
@@ -688,18 +690,27 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                 accA += fs_term((CT)e.c, (uint32_t)__builtin_popcount(a32 & e.m));
                 accB += fs_term((CT)e.c, (uint32_t)__builtin_popcount(b32 & e.m));
             }
+            const uint32_t ahi = (uint32_t)(WA[0] >> 32), bhi = (uint32_t)(WB[0] >> 32);
             const FsFull *pfu = s_full + lane;
 #pragma unroll 4
             for (int it = 0; it < nfull; ++it) {
                 const FsFull e = pfu[64 * it];
-                accA += fs_term((CT)e.c, (uint32_t)__popcll(WA[0] & e.m));
-                accB += fs_term((CT)e.c, (uint32_t)__popcll(WB[0] & e.m));
+                // parity of a 64-bit AND = parity of its two words XOR-folded (one popcount)
+                const uint32_t mlo = (uint32_t)e.m, mhi = (uint32_t)(e.m >> 32);
+                accA += fs_term((CT)e.c, (uint32_t)__builtin_popcount(and_xor(a32, mlo, ahi & mhi)));
+                accB += fs_term((CT)e.c, (uint32_t)__builtin_popcount(and_xor(b32, mlo, bhi & mhi)));
             }
             for (int it = 0; it < ntail; ++it) {
                 const FsTail e = s_tail[64 * it + lane];
                 const unsigned long long a1 = e.word == 1 ? WA[1] : WA[2], b1 = e.word == 1 ? WB[1] : WB[2];
-                accA += fs_term((CT)e.c, (uint32_t)__popcll((WA[0] & e.m0) ^ (a1 & e.mk)));
-                accB += fs_term((CT)e.c, (uint32_t)__popcll((WB[0] & e.m0) ^ (b1 & e.mk)));
+                const uint32_t m0l = (uint32_t)e.m0, m0h = (uint32_t)(e.m0 >> 32);
+                const uint32_t mkl = (uint32_t)e.mk, mkh = (uint32_t)(e.mk >> 32);
+                const uint32_t fa = and_xor(a32, m0l, and_xor(ahi, m0h, and_xor((uint32_t)a1, mkl,
+                                                                                (uint32_t)(a1 >> 32) & mkh)));
+                const uint32_t fb = and_xor(b32, m0l, and_xor(bhi, m0h, and_xor((uint32_t)b1, mkl,
+                                                                                (uint32_t)(b1 >> 32) & mkh)));
+                accA += fs_term((CT)e.c, (uint32_t)__builtin_popcount(fa));
+                accB += fs_term((CT)e.c, (uint32_t)__builtin_popcount(fb));
             }
             const Real S = wave_sum(accA - accB);
             sum = sa == 0 ? (Real)2 * S : (Real)-2 * S;
