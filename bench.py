@@ -36,7 +36,7 @@ PUBLISHED_H100 = 4_605_830.0      # reference: CE attempts/s at N = 1,048,576, R
 # HBM bytes per swap attempt moved by the chunk kernel, from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per
 # dispatch (factors from a 1 GiB device copy): profiles/ce_chunk_spec_kernel_fcc64cubic_1m_r256.md and
 # profiles/nn_chunk_kernel_ptni_fcc16cubic_r32.md
-MEASURED_TRAFFIC_PER_ATTEMPT = {"ce_1m": 8406.0, "nce_ptni_16k": 43640.0}
+MEASURED_TRAFFIC_PER_ATTEMPT = {"ce_1m": 3473.0, "nce_ptni_16k": 43640.0}
 
 CONFIGS = {
     "ce_1m": dict(n=64, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
@@ -226,7 +226,7 @@ def main():
                 "data": "synthetic (seeded random occupations / weights)",
                 "config": {"workload": f"NeuralCE PT swap MC ({cfg['nce']}), fcc {cfg['n']}^3 cubic = {res['N']} "
                                        f"sites, {res['R']} replicas", "chunk_steps": args.chunk},
-                "roofline": {"bound": "mfma" if res["prec"] == 2 else "valu", "achieved": res["achieved"],
+                "roofline": {"bound": "mfma", "achieved": res["achieved"],
                              "peak": res["peak"],
                              "traffic": (MEASURED_TRAFFIC_PER_ATTEMPT[args.config] * res["R"] * args.chunk
                                          if args.config in MEASURED_TRAFFIC_PER_ATTEMPT else None),
