@@ -213,6 +213,33 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// DPP copy for scans: lanes whose source lies outside the row, or whose row / bank the masks leave out, read 0
+template <int CTRL, int ROWS, int BANKS>
+__device__ inline double dpp_f64_z(double v) {
+    const int64_t b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, ROWS, BANKS, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, BANKS, true);
+    return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+// inclusive prefix sum over the 64 lanes: row_shr 1, 2, 4, 8 within 16-lane rows, then row_bcast:15 into rows 1
+// and 3 and row_bcast:31 into rows 2 and 3; fixed order (deterministic)
+__device__ inline double wave_prefix_sum(double v) {
+    v += dpp_f64_z<0x111, 0xF, 0xF>(v);
+    v += dpp_f64_z<0x112, 0xF, 0xF>(v);
+    v += dpp_f64_z<0x114, 0xF, 0xF>(v);
+    v += dpp_f64_z<0x118, 0xF, 0xF>(v);
+    v += dpp_f64_z<0x142, 0xA, 0xF>(v);
+    v += dpp_f64_z<0x143, 0xC, 0xF>(v);
+    return v;
+}
+__device__ inline double wave_min(double v) {
+    v = fmin(v, dpp_f64<0xB1>(v));
+    v = fmin(v, dpp_f64<0x4E>(v));
+    v = fmin(v, dpp_f64<0x141>(v));
+    v = fmin(v, dpp_f64<0x140>(v));
+    return fmin(fmin(readlane_f64(v, 0), readlane_f64(v, 16)), fmin(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
 // wave sum, result uniform in every lane; fixed order (deterministic)
 __device__ inline double wave_sum(double v) {
     v += dpp_f64<0xB1>(v);
@@ -869,6 +896,9 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             rec.dE = dE; rec.lu = lu; rec.sa = sa; rec.sb = sb; rec.valid = valid; rec.conflict = conflict;
             s_rec[w] = rec;
         }
+        // wave 0 finishes its evaluation first: the next group is formed here from the species as loaded and
+        // formed again after the commits only if one of them touches a site of the next batch
+        if (w == 0 && t1 < n_steps) form_group(t1);
         lds_barrier();
         // 3. decisions.  Lane i of every wave holds step i's record; the tentative decisions are taken in parallel,
         // then every step whose windows hold a site changed by an accepted earlier step of the group is
@@ -954,13 +984,15 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 const int nc = __popcll(accm);
+                bool hit = false;
 #pragma unroll 4
                 for (int c = 0; c < nc; ++c) {
                     const int4 e = s_commit[c];
+                    hit |= (npia == e.x) | (npia == e.y) | (npib == e.x) | (npib == e.y);
                     nsa = npia == e.x ? e.z : (npia == e.y ? e.w : nsa);
                     nsb = npib == e.x ? e.z : (npib == e.y ? e.w : nsb);
                 }
-                form_group(t1);
+                if (__ballot(hit) != 0ull) form_group(t1);
             }
         }
         if (w == W - 1) {
@@ -972,29 +1004,20 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             const bool in = lane < (int)nstep;
             const bool acc_i = in && ((gvm >> lane) & 1ull) && ((accm >> kl) & 1ull);
             const double dk = __shfl(ri.dE, kl < 64 ? kl : 0, 64);
-            double P = acc_i ? dk : 0.0;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const double v = __shfl_up(P, (unsigned)o, 64);
-                if (lane >= o) P += v;
-            }
-            const double Ei = E + P;
-            double mn = in ? Ei : E;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) mn = fmin(mn, __shfl_xor(mn, o, 64));
-            emin = fmin(emin, mn);
+            const double Ei = E + wave_prefix_sum(acc_i ? dk : 0.0);
+            emin = fmin(emin, wave_min(in ? Ei : E));
             if (accumulate) {
                 const double x = in ? Ei - eref : 0.0;
                 s1 += wave_sum(x);
                 s2 += wave_sum(x * x);
             }
             while (traj && next_sample <= t0 + nstep) {
-                const double Es = __shfl(Ei, (int)(next_sample - t0 - 1), 64);
+                const double Es = readlane_f64(Ei, (int)(next_sample - t0 - 1));
                 if (lane == 0) traj[n_sampled * n_rep + slot_r] = Es;
                 next_sample += sample_interval;
                 ++n_sampled;
             }
-            E = __shfl(Ei, (int)nstep - 1, 64);
+            E = readlane_f64(Ei, (int)nstep - 1);
         }
         __syncthreads();                               // occupation writes and the next group
         t0 = t1;
