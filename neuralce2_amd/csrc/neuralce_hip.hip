@@ -669,6 +669,35 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     const int64_t n_cells = mg.n_cells;
     const int d0 = mg.d0, d1 = mg.d1, d2 = mg.d2;
     const uint32_t m12 = mg.div12_m, s12 = mg.div12_s, m2 = mg.div2_m, sh2 = mg.div2_s;
+    // window offsets as signed representatives in (-d/2, d/2] and their box: the window of a site at least the box
+    // away from every boundary does not wrap, and its slot addresses are the site plus lin[j]
+    int lin[NS];
+    int blo0 = 0, bhi0 = 0, blo1 = 0, bhi1 = 0, blo2 = 0, bhi2 = 0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        lin[j] = 0;
+        if (wok[j]) {
+            const int o0 = wofs[j].y > d0 / 2 ? wofs[j].y - d0 : wofs[j].y;
+            const int o1 = wofs[j].z > d1 / 2 ? wofs[j].z - d1 : wofs[j].z;
+            const int o2 = wofs[j].w > d2 / 2 ? wofs[j].w - d2 : wofs[j].w;
+            lin[j] = (o0 * d1 + o1) * d2 + o2;
+            blo0 = min(blo0, o0); bhi0 = max(bhi0, o0);
+            blo1 = min(blo1, o1); bhi1 = max(bhi1, o1);
+            blo2 = min(blo2, o2); bhi2 = max(bhi2, o2);
+        }
+    }
+    for (int o = 1; o < 64; o <<= 1) {
+        blo0 = min(blo0, __shfl_xor(blo0, o, 64)); bhi0 = max(bhi0, __shfl_xor(bhi0, o, 64));
+        blo1 = min(blo1, __shfl_xor(blo1, o, 64)); bhi1 = max(bhi1, __shfl_xor(bhi1, o, 64));
+        blo2 = min(blo2, __shfl_xor(blo2, o, 64)); bhi2 = max(bhi2, __shfl_xor(bhi2, o, 64));
+    }
+    blo0 = __builtin_amdgcn_readfirstlane(blo0); bhi0 = __builtin_amdgcn_readfirstlane(bhi0);
+    blo1 = __builtin_amdgcn_readfirstlane(blo1); bhi1 = __builtin_amdgcn_readfirstlane(bhi1);
+    blo2 = __builtin_amdgcn_readfirstlane(blo2); bhi2 = __builtin_amdgcn_readfirstlane(bhi2);
+    auto interior = [&](int c0, int c1, int c2) __attribute__((always_inline)) -> bool {
+        return c0 + blo0 >= 0 && c0 + bhi0 < d0 && c1 + blo1 >= 0 && c1 + bhi1 < d1 && c2 + blo2 >= 0 &&
+               c2 + bhi2 < d2;
+    };
     const uint32_t d12 = (uint32_t)(d1 * d2);
     const double factor = mg.factor;
 
@@ -850,14 +879,27 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         const int b0 = (int)udiv_magic((uint32_t)ib, m12, s12, d12), brem = ib - b0 * (int)d12;
         const int b1 = (int)udiv_magic((uint32_t)brem, m2, sh2, (uint32_t)d2), b2 = brem - b1 * d2;
         int sA[NS], sB[NS], pA[NS], pB[NS];
+        if (interior(a0, a1, a2)) {
 #pragma unroll
-        for (int j = 0; j < NS; ++j) {
-            int k0 = a0 + wofs[j].y, k1 = a1 + wofs[j].z, k2 = a2 + wofs[j].w;
-            k0 -= (k0 >= d0) ? d0 : 0; k1 -= (k1 >= d1) ? d1 : 0; k2 -= (k2 >= d2) ? d2 : 0;
-            sA[j] = wok[j] ? (k0 * d1 + k1) * d2 + k2 : -1;
-            k0 = b0 + wofs[j].y; k1 = b1 + wofs[j].z; k2 = b2 + wofs[j].w;
-            k0 -= (k0 >= d0) ? d0 : 0; k1 -= (k1 >= d1) ? d1 : 0; k2 -= (k2 >= d2) ? d2 : 0;
-            sB[j] = wok[j] ? (k0 * d1 + k1) * d2 + k2 : -1;
+            for (int j = 0; j < NS; ++j) sA[j] = wok[j] ? ia + lin[j] : -1;
+        } else {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                int k0 = a0 + wofs[j].y, k1 = a1 + wofs[j].z, k2 = a2 + wofs[j].w;
+                k0 -= (k0 >= d0) ? d0 : 0; k1 -= (k1 >= d1) ? d1 : 0; k2 -= (k2 >= d2) ? d2 : 0;
+                sA[j] = wok[j] ? (k0 * d1 + k1) * d2 + k2 : -1;
+            }
+        }
+        if (interior(b0, b1, b2)) {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) sB[j] = wok[j] ? ib + lin[j] : -1;
+        } else {
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                int k0 = b0 + wofs[j].y, k1 = b1 + wofs[j].z, k2 = b2 + wofs[j].w;
+                k0 -= (k0 >= d0) ? d0 : 0; k1 -= (k1 >= d1) ? d1 : 0; k2 -= (k2 >= d2) ? d2 : 0;
+                sB[j] = wok[j] ? (k0 * d1 + k1) * d2 + k2 : -1;
+            }
         }
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
