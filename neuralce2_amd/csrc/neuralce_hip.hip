@@ -611,6 +611,16 @@ __host__ __device__ inline SpecLds<Real> spec_lds_layout(int J, int Wcap, int wa
     return L;
 }
 
+// Step rank of wave w within a group.  Wave w runs on SIMD w % 4 and the rank-k step scans the k earlier steps for
+// conflicts, so the ranks are spread such that every SIMD's ranks add up to the same total.
+template <int W>
+__device__ inline int spec_rank(int w) {
+    const int s = w & 3, slot = w >> 2;
+    if constexpr (W == 4) return w;
+    else if constexpr (W == 8) return slot == 0 ? s : 7 - s;
+    else return slot == 0 ? s : slot == 1 ? 7 - s : slot == 2 ? 8 + s : 15 - s;
+}
+
 template <typename Real, int W, bool INJ>
 __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *__restrict__ models,
                                                                const nce_pt_job *__restrict__ jobs,
@@ -624,6 +634,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     const int r = it.y;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int kw = spec_rank<W>(w);                   // this wave's step in every group
     const int J = mg.J, Wcap = mg.Wcap, Wn = mg.W_b[0], Jb0 = mg.J_b[0];
     const int nlo = mg.fs_nlo, nfull = mg.fs_nfull, ntail = mg.fs_ntail;
     const SpecLds<Real> L = spec_lds_layout<Real>(J, Wcap, W, nlo, nfull, ntail);
@@ -864,12 +875,12 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         const int64_t nstep = (int64_t)s_meta[1];
         const int64_t t1 = t0 + nstep;                 // first step of the next group
         const int nW = __popcll(gvm);                // different-species swaps of the group, one per wave
-        const bool active = w < nW;
+        const bool active = kw < nW;
         // 1. this wave's swap and its window loads
         int ia = 0, ib = 0;
         double u = 0.0;
         if (active) {
-            const SpecProp pp = s_pp[w];
+            const SpecProp pp = s_pp[kw];
             ia = pp.ia; ib = pp.ib; u = pp.u;
         }
         ia = __builtin_amdgcn_readfirstlane(ia);
@@ -907,11 +918,11 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             pB[j] = wok[j] ? (int)occ[sB[j]] : 1;
         }
         if (w == 0 && t1 < n_steps) draw(t1);
-        if (lane == 0) s_prop[w] = make_int2(ia, ib);
+        if (lane == 0) s_prop[kw] = make_int2(ia, ib);
         lds_barrier();
         // 2. conflicts with the earlier steps of the group, centres, near flag, energy change
         int conflict = 0;
-        for (int q = 0; q < w; ++q) {
+        for (int q = 0; q < kw; ++q) {
             const int2 pq = s_prop[q];
             bool hit = false;
 #pragma unroll
@@ -936,11 +947,21 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         if (lane == 0) {
             SpecRec rec;
             rec.dE = dE; rec.lu = lu; rec.sa = sa; rec.sb = sb; rec.valid = valid; rec.conflict = conflict;
-            s_rec[w] = rec;
+            s_rec[kw] = rec;
         }
-        // wave 0 finishes its evaluation first: the next group is formed here from the species as loaded and
-        // formed again after the commits only if one of them touches a site of the next batch
-        if (w == 0 && t1 < n_steps) form_group(t1);
+        // wave 0 finishes its evaluation first: the next group is formed here from the species as loaded.  A
+        // commit of this group is a swap of one of its proposals, so unless a site of the next batch is one of
+        // their sites (rare) no commit can change it and the group stands.
+        bool spec_hit = false;
+        if (w == 0 && t1 < n_steps) {
+            form_group(t1);
+            bool h = false;
+            for (int q = 0; q < nW; ++q) {
+                const int2 pq = s_prop[q];
+                h |= (npia == pq.x) | (npia == pq.y) | (npib == pq.x) | (npib == pq.y);
+            }
+            spec_hit = __ballot(h) != 0ull;
+        }
         lds_barrier();
         // 3. decisions.  Lane i of every wave holds step i's record; the tentative decisions are taken in parallel,
         // then every step whose windows hold a site changed by an accepted earlier step of the group is
@@ -966,7 +987,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                     }
             }
             lds_barrier();
-            if (w == f) {
+            if (kw == f) {
                 const int n_commit = __popcll(accm & ((1ull << f) - 1ull));
                 int qA[NS], qB[NS];
 #pragma unroll
@@ -1017,7 +1038,7 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                         occ[pi.y] = (int8_t)q.sa;
                     }
             }
-            if (t1 < n_steps) {
+            if (t1 < n_steps && spec_hit) {
                 // the group's commits in step order, then every lane of the next batch patches its species
                 const int crank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(accm >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)accm, 0u));
