@@ -47,7 +47,11 @@ CONFIGS = {
     "nce_16k": dict(n=16, x=0.5, R=64, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="bf16", nce="silu"),
     # NeuralCE: the reference's shipped PtNi model (depth 1, width 16, 3 shells) at 16,384 sites, 32 replicas
     "nce_ptni_16k": dict(n=16, x=0.5, R=32, Tmin=700.0, Tmax=1100.0, exchange=5000, precision="mixed", nce="ptni"),
+    "nce_ptni_1m": dict(n=64, x=0.5, R=32, Tmin=700.0, Tmax=1100.0, exchange=5000, precision="mixed", nce="ptni"),
+    "nce_ptni_10m": dict(n=136, x=0.5, R=8, Tmin=700.0, Tmax=1100.0, exchange=5000, precision="mixed", nce="ptni"),
 }
+# reference NeuralCE local-update attempts/s on H100 at the same N and R (results/bench_stage/gpu_conv_*.json)
+PUBLISHED_H100_NCE = {"nce_ptni_16k": 159_250.0, "nce_ptni_1m": 84_585.0, "nce_ptni_10m": 14_615.0}
 
 
 def nce_bench(args, cfg):
@@ -221,9 +225,12 @@ def main():
         res = nce_bench(args, cfg)
         line = {"metric": "MC swap attempts/s (all replicas)", "value": res["value"], "unit": "attempts/s",
                 "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["dt"] / args.steps * 1e3,
-                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": (res["value"] / PUBLISHED_H100_NCE[args.config]
+                                if args.config in PUBLISHED_H100_NCE else None),
                 "dtype": ["f64", "f32", "bf16 mfma / f32"][res["prec"]],
-                "data": "synthetic (seeded random occupations / weights)",
+                "data": ("synthetic (seeded random occupations; the shipped PtNi model's weights)" if cfg["nce"] == "ptni"
+                         else "synthetic (seeded random occupations and weights)"),
                 "config": {"workload": f"NeuralCE PT swap MC ({cfg['nce']}), fcc {cfg['n']}^3 cubic = {res['N']} "
                                        f"sites, {res['R']} replicas", "chunk_steps": args.chunk},
                 "roofline": {"bound": "mfma", "achieved": res["achieved"],
