@@ -710,6 +710,19 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                c2 + bhi2 < d2;
     };
     const uint32_t d12 = (uint32_t)(d1 * d2);
+    auto coords = [&](int x, int &c0, int &c1, int &c2) __attribute__((always_inline)) {
+        c0 = (int)udiv_magic((uint32_t)x, m12, s12, d12);
+        const int rem = x - c0 * (int)d12;
+        c1 = (int)udiv_magic((uint32_t)rem, m2, sh2, (uint32_t)d2);
+        c2 = rem - c1 * d2;
+    };
+    // site y (coordinates y0..y2) inside the offset box of the window of a site with coordinates c0..c2: necessary
+    // for y to be a slot of that window
+    auto in_box = [&](int y0, int y1, int y2, int c0, int c1, int c2) __attribute__((always_inline)) -> bool {
+        int e0 = y0 - c0, e1 = y1 - c1, e2 = y2 - c2;
+        e0 += e0 < 0 ? d0 : 0; e1 += e1 < 0 ? d1 : 0; e2 += e2 < 0 ? d2 : 0;
+        return (e0 <= bhi0 || e0 >= d0 + blo0) && (e1 <= bhi1 || e1 >= d1 + blo1) && (e2 <= bhi2 || e2 >= d2 + blo2);
+    };
     const double factor = mg.factor;
 
     NCE_GLOBAL int8_t *occ = gptr(job.occ) + (int64_t)r * job.n_sites;
@@ -920,15 +933,32 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         if (w == 0 && t1 < n_steps) draw(t1);
         if (lane == 0) s_prop[kw] = make_int2(ia, ib);
         lds_barrier();
-        // 2. conflicts with the earlier steps of the group, centres, near flag, energy change
+        // 2. conflicts with the earlier steps of the group, centres, near flag, energy change.  An earlier step's
+        // site can be a slot of this step's windows only inside their offset box: one lane-parallel box test over
+        // the 2 kw earlier sites, then an exact comparison of the slots for the steps that pass it (rare).
         int conflict = 0;
-        for (int q = 0; q < kw; ++q) {
-            const int2 pq = s_prop[q];
-            bool hit = false;
+        if (kw > 0) {
+            int a0, a1, a2, b0, b1, b2;
+            coords(ia, a0, a1, a2);
+            coords(ib, b0, b1, b2);
+            bool inb = false;
+            if (lane < 2 * kw) {
+                const int2 pq = s_prop[lane >> 1];
+                int y0, y1, y2;
+                coords((lane & 1) ? pq.y : pq.x, y0, y1, y2);
+                inb = in_box(y0, y1, y2, a0, a1, a2) || in_box(y0, y1, y2, b0, b1, b2);
+            }
+            uint64_t cand = __ballot(inb);
+            while (cand) {
+                const int q = __builtin_ctzll(cand) >> 1;
+                cand &= ~(3ull << (2 * q));
+                const int2 pq = s_prop[q];
+                bool hit = false;
 #pragma unroll
-            for (int j = 0; j < NS; ++j)
-                hit |= (sA[j] == pq.x) | (sA[j] == pq.y) | (sB[j] == pq.x) | (sB[j] == pq.y);
-            if (__ballot(hit) != 0ull) conflict |= 1 << q;
+                for (int j = 0; j < NS; ++j)
+                    hit |= (sA[j] == pq.x) | (sA[j] == pq.y) | (sB[j] == pq.x) | (sB[j] == pq.y);
+                if (__ballot(hit) != 0ull) conflict |= 1 << q;
+            }
         }
         bool nr = false;
 #pragma unroll
