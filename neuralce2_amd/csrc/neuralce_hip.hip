@@ -705,6 +705,12 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
     blo0 = __builtin_amdgcn_readfirstlane(blo0); bhi0 = __builtin_amdgcn_readfirstlane(bhi0);
     blo1 = __builtin_amdgcn_readfirstlane(blo1); bhi1 = __builtin_amdgcn_readfirstlane(bhi1);
     blo2 = __builtin_amdgcn_readfirstlane(blo2); bhi2 = __builtin_amdgcn_readfirstlane(bhi2);
+    // a window holding an image of its own centre (a slot other than 0 at offset 0 modulo the cell): small cells only
+    bool selfimg = false;
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+        selfimg |= wok[j] && (lane + 64 * j) > 0 && wofs[j].y == 0 && wofs[j].z == 0 && wofs[j].w == 0;
+    const bool self_image = __ballot(selfimg) != 0ull;
     auto interior = [&](int c0, int c1, int c2) __attribute__((always_inline)) -> bool {
         return c0 + blo0 >= 0 && c0 + bhi0 < d0 && c1 + blo1 >= 0 && c1 + bhi1 < d1 && c2 + blo2 >= 0 &&
                c2 + bhi2 < d2;
@@ -938,9 +944,6 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
         // the 2 kw earlier sites, then an exact comparison of the slots for the steps that pass it (rare).
         int conflict = 0;
         if (kw > 0) {
-            int a0, a1, a2, b0, b1, b2;
-            coords(ia, a0, a1, a2);
-            coords(ib, b0, b1, b2);
             bool inb = false;
             if (lane < 2 * kw) {
                 const int2 pq = s_prop[lane >> 1];
@@ -960,11 +963,16 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                 if (__ballot(hit) != 0ull) conflict |= 1 << q;
             }
         }
-        bool nr = false;
+        // near: a slot other than a window's centre is one of the two sites (exact test only when the other site
+        // is inside the window's box, or the windows hold images of their centres)
+        bool near = false;
+        if (self_image || in_box(b0, b1, b2, a0, a1, a2) || in_box(a0, a1, a2, b0, b1, b2)) {
+            bool nr = false;
 #pragma unroll
-        for (int j = 0; j < NS; ++j)
-            if (wok[j] && (lane + 64 * j) > 0) nr |= (sA[j] == ia) | (sA[j] == ib) | (sB[j] == ib) | (sB[j] == ia);
-        const bool near = __ballot(nr) != 0ull;
+            for (int j = 0; j < NS; ++j)
+                if (wok[j] && (lane + 64 * j) > 0) nr |= (sA[j] == ia) | (sA[j] == ib) | (sB[j] == ib) | (sB[j] == ia);
+            near = __ballot(nr) != 0ull;
+        }
         const int sa = __builtin_amdgcn_readfirstlane(pA[0]), sb = __builtin_amdgcn_readfirstlane(pB[0]);
         const bool valid = active && sa != sb;
         double dE = 0.0, lu = 0.0;
