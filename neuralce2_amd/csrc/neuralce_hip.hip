@@ -849,8 +849,8 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             const Proposal p = draw_proposal_1b(seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)n_cells);
             npia = (int)p.ia; npib = (int)p.ib; npu = p.u;
         }
-        nsa = act ? (int)occ[npia] : 0;
-        nsb = act ? (int)occ[npib] : 0;
+        nsa = act ? (int)occ[(uint32_t)npia] : 0;
+        nsb = act ? (int)occ[(uint32_t)npib] : 0;
     };
     // s_pp[k] = the k-th different-species swap of the batch (k < W), the proposal of wave k
     auto form_group = [&](int64_t tb) __attribute__((always_inline)) {
@@ -931,10 +931,13 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
                 sB[j] = wok[j] ? (k0 * d1 + k1) * d2 + k2 : -1;
             }
         }
+        // every lane loads (a lane without a window slot reads the centre's byte, which shares a line with others),
+        // so the loads need no exec-mask branches
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
-            pA[j] = wok[j] ? (int)occ[sA[j]] : 1;
-            pB[j] = wok[j] ? (int)occ[sB[j]] : 1;
+            const int va = (int)occ[(uint32_t)(wok[j] ? sA[j] : ia)], vb = (int)occ[(uint32_t)(wok[j] ? sB[j] : ib)];
+            pA[j] = wok[j] ? va : 1;
+            pB[j] = wok[j] ? vb : 1;
         }
         if (w == 0 && t1 < n_steps) draw(t1);
         if (lane == 0) s_prop[kw] = make_int2(ia, ib);
@@ -1064,8 +1067,8 @@ __global__ __launch_bounds__(64 * W) void ce_chunk_spec_kernel(const ModelDev *_
             const int2 pl = s_prop[lane < nW ? lane : 0];
             if (__ballot(mine && (ri.conflict & (int)accm) != 0) == 0ull) {
                 if (mine) {
-                    occ[pl.x] = (int8_t)ri.sb;
-                    occ[pl.y] = (int8_t)ri.sa;
+                    occ[(uint32_t)pl.x] = (int8_t)ri.sb;
+                    occ[(uint32_t)pl.y] = (int8_t)ri.sa;
                 }
             } else if (lane == 0) {
                 for (int i = 0; i < nW; ++i)
