@@ -283,6 +283,11 @@ __device__ inline int nn_block_scan(int *a, int n, double *red_d) {
     return total;
 }
 
+// four consecutive channels (16-byte aligned rows: every cache / scratch region starts at a multiple of 4 elements)
+template <typename Real>
+struct alignas(16) NNv4 {
+    Real v[4];
+};
 typedef __bf16 nn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float nn_floatx4 __attribute__((ext_vector_type(4)));
 
@@ -527,6 +532,74 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const bool last = l + 1 == L;
                 const int og = D * (1 + P);
                 constexpr int U = 4;                     // items per thread in flight together
+                if (sizeof(Real) == 4 && (D & 3) == 0) {
+                // four channels per item (FP32): every hit's projection block is read 16 bytes at a time
+                using V4 = NNv4<Real>;
+                constexpr int U2 = 2;                    // items (four channels each) per thread in flight together
+                const int D4 = D >> 2;
+                for (int k0 = tid; k0 < nb * D4; k0 += U2 * NN_BS) {
+                    V4 p0[U2], g0[U2];
+                    int sv[U2], dv[U2];
+#pragma unroll
+                    for (int u = 0; u < U2; ++u) {
+                        const int k = k0 + u * NN_BS;
+                        const int kk = k < nb * D4 ? k : 0;
+                        sv[u] = kk / D4; dv[u] = (kk - sv[u] * D4) * 4;
+                        const int64_t o = (int64_t)slot_site[sv[u]] * D + dv[u];
+                        p0[u] = *(const V4 *)(cpre + o);
+                        if (m.has_gate) g0[u] = *(const V4 *)(cgpre + o);
+                        else g0[u] = V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
+                    }
+#pragma unroll
+                    for (int u = 0; u < U2; ++u) {
+                        const int k = k0 + u * NN_BS;
+                        if (k >= nb * D4) break;
+                        const int s = sv[u], d = dv[u];
+                        V4 dp = s < prev ? *(const V4 *)(pr + (int64_t)s * M + d) : V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
+                        V4 dg = V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
+                        const int j0 = s == 0 ? 0 : cnt[s - 1], j1 = cnt[s];
+                        auto at = [&](int h) -> int64_t { return (int64_t)(h >> 8) * M + D * (h & 0xff) + d; };
+                        auto rd = [&](int64_t a) -> V4 { return *(const V4 *)(pr + a); };
+                        int j = j0;
+                        for (; j + 4 <= j1; j += 4) {
+                            const int64_t a0 = at(list[j]), a1 = at(list[j + 1]), a2 = at(list[j + 2]),
+                                          a3 = at(list[j + 3]);
+                            const V4 x0 = rd(a0 + D), x1 = rd(a1 + D), x2 = rd(a2 + D), x3 = rd(a3 + D);
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) dp.v[e] += (x0.v[e] + x1.v[e]) + (x2.v[e] + x3.v[e]);
+                            if (m.has_gate) {
+                                const V4 y0 = rd(a0 + og), y1 = rd(a1 + og), y2 = rd(a2 + og), y3 = rd(a3 + og);
+#pragma unroll
+                                for (int e = 0; e < 4; ++e) dg.v[e] += (y0.v[e] + y1.v[e]) + (y2.v[e] + y3.v[e]);
+                            }
+                        }
+                        for (; j < j1; ++j) {
+                            const int64_t a = at(list[j]);
+                            const V4 x = rd(a + D);
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) dp.v[e] += x.v[e];
+                            if (m.has_gate) {
+                                const V4 y = rd(a + og);
+#pragma unroll
+                                for (int e = 0; e < 4; ++e) dg.v[e] += y.v[e];
+                            }
+                        }
+                        V4 dhv;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            dhv.v[e] = nn_act<Real>(m, p0[u].v[e] + dp.v[e], g0[u].v[e] + dg.v[e]) -
+                                       nn_act<Real>(m, p0[u].v[e], g0[u].v[e]);
+                        *(V4 *)(dpre + 4 * (int64_t)k) = dp;
+                        if (m.has_gate) *(V4 *)(dgpre + 4 * (int64_t)k) = dg;
+                        if (last) {
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) part += (double)(dhv.v[e] * hw[d + e]);
+                        } else {
+                            *(V4 *)(dh + 4 * (int64_t)k) = dhv;
+                        }
+                    }
+                }
+                } else
                 for (int k0 = tid; k0 < nb * D; k0 += U * NN_BS) {
                     Real p0[U], g0[U];
                     int sv[U], dv[U];
@@ -765,11 +838,12 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     int hc = 64;
     while (hc < m.ball_cap[m.L] + m.ball_cap[m.L] / 2) hc <<= 1;
     m.hash_cap = hc;
-    // per-replica cache layout
-    int64_t o = N * m.C0;
+    // per-replica cache layout; every region starts at a multiple of 4 elements (16-byte rows of four channels)
+    auto pad4 = [](int64_t v) { return (v + 3) & ~(int64_t)3; };
+    int64_t o = pad4(N * m.C0);
     for (int l = 0; l < m.L; ++l) {
-        m.off_pre[l] = o; o += N * m.dd[l];
-        m.off_gpre[l] = o; if (m.has_gate) o += N * m.dd[l];
+        m.off_pre[l] = o; o = pad4(o + N * m.dd[l]);
+        m.off_gpre[l] = o; if (m.has_gate) o = pad4(o + N * m.dd[l]);
     }
     m.cache_elems = o;
     // per-replica scratch layout (elements of the precision's type; int arrays use one element per int)
@@ -779,16 +853,16 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
         hit_max = std::max(hit_max, (int64_t)m.ball_cap[l] * std::max(n_part, 1));
         dh_max = std::max(dh_max, (int64_t)m.ball_cap[l + 1] * m.dd[l]);
     }
-    m.soff_dh0 = s2; s2 += (int64_t)m.ball_cap[0] * m.C0;
-    m.soff_pr = s2; s2 += pr_max;
-    m.soff_tgt = s2; s2 += hit_max;
-    m.soff_list = s2; s2 += hit_max;
+    m.soff_dh0 = s2; s2 = pad4(s2 + (int64_t)m.ball_cap[0] * m.C0);
+    m.soff_pr = s2; s2 = pad4(s2 + pr_max);
+    m.soff_tgt = s2; s2 = pad4(s2 + hit_max);
+    m.soff_list = s2; s2 = pad4(s2 + hit_max);
     for (int l = 0; l < m.L; ++l) {
-        m.soff_dpre[l] = s2; s2 += (int64_t)m.ball_cap[l + 1] * m.dd[l];
-        m.soff_dgpre[l] = s2; if (m.has_gate) s2 += (int64_t)m.ball_cap[l + 1] * m.dd[l];
+        m.soff_dpre[l] = s2; s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
+        m.soff_dgpre[l] = s2; if (m.has_gate) s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
     }
-    m.soff_dh[0] = s2; s2 += dh_max;
-    m.soff_dh[1] = s2; s2 += dh_max;
+    m.soff_dh[0] = s2; s2 = pad4(s2 + dh_max);
+    m.soff_dh[1] = s2; s2 = pad4(s2 + dh_max);
     m.scratch_elems = s2;
     std::vector<float> mu_f(m.C0), sd_f(m.C0), lin_f(m.C0), w_f(wts.size()), hw_f(std::max(DL, 1));
     for (int i = 0; i < m.C0; ++i) { mu_f[i] = (float)d->mu[i]; sd_f[i] = (float)d->sd[i]; lin_f[i] = (float)d->lin[i]; }
