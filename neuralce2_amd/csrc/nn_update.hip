@@ -332,6 +332,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int64_t nacc = 0;
     const int64_t inj_base = (int64_t)r * n_steps;
     int counts[NN_MAXL + 1];
+    bool w_staged = false;                             // the projection matrix in LDS is the current layer's
 
     for (int64_t t = 0; t < n_steps; ++t) {
         int64_t ia, ib;
@@ -413,11 +414,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     // Lane l holds A[row l%16][k 8(l/16)..+7], B[k 8(l/16)..+7][col l%16], C[rows 4(l/16)..+3][col l%16].
                     const int Kp = (Cin + 31) & ~31;
                     __bf16 *wt = (__bf16 *)wl;
-                    for (int k = tid; k < M * Kp; k += NN_BS) {
-                        const int j = k / Kp, c = k - j * Kp;
-                        wt[k] = c < Cin ? (__bf16)Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j] : (__bf16)0.0f;
+                    if (!w_staged) {
+                        for (int k = tid; k < M * Kp; k += NN_BS) {
+                            const int j = k / Kp, c = k - j * Kp;
+                            wt[k] = c < Cin ? (__bf16)Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j] : (__bf16)0.0f;
+                        }
+                        __syncthreads();
                     }
-                    __syncthreads();
                     const int lane = tid & 63, wv = tid >> 6;
                     const int nrt = (prev + 15) >> 4, nct = M >> 4;
                     constexpr int CT = 4;                // column tiles per pass (A fragment loaded once per pass)
@@ -457,11 +460,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     }
                 } else {
                 // the layer's matrix (used rows only) staged in LDS, 4 x 4 register tiles
-                for (int k = tid; k < Cin * M; k += NN_BS) {
-                    const int c = k / M, j = k - c * M;
-                    wl[k] = Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j];
+                if (!w_staged) {
+                    for (int k = tid; k < Cin * M; k += NN_BS) {
+                        const int c = k / M, j = k - c * M;
+                        wl[k] = Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j];
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
                 const int nqb = (prev + NN_RB - 1) / NN_RB, ncb = (M + 3) / 4;
                 for (int k = tid; k < nqb * ncb; k += NN_BS) {
                     const int qb = k / ncb, j0 = (k - qb * ncb) * 4;
@@ -643,6 +648,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 }
                 __syncthreads();
             }
+            w_staged = L == 1;                           // one layer: its matrix stays in LDS for the launch
             const double diff = nn_block_sum<double>(part, red);
             const double dE = (m.readout == 0 ? diff / (double)m.nA : diff);
             const double dE_eV = dE * m.de_scale;
