@@ -458,6 +458,48 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             }
                         }
                     }
+                } else if (sizeof(Real) == 4 && m.mfma_ok) {
+                    // FP32 MFMA (v_mfma_f32_16x16x4_f32): the layer's matrix staged in LDS as [c][j]; one 16 x 16
+                    // output tile per wave at a time, CT column tiles per pass.  Lane l holds A[row l%16][k l/16],
+                    // B[k l/16][col l%16], C[rows 4(l/16)..+3][col l%16].
+                    if (!w_staged) {
+                        for (int k = tid; k < Cin * M; k += NN_BS) {
+                            const int c = k / M, j = k - c * M;
+                            wl[k] = Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j];
+                        }
+                        __syncthreads();
+                    }
+                    const int lane = tid & 63, wv = tid >> 6;
+                    const int nrt = (prev + 15) >> 4, nct = M >> 4;
+                    constexpr int CT = 4;
+                    for (int task = wv; task < nrt * ((nct + CT - 1) / CT); task += NN_BS / 64) {
+                        const int rt = task % nrt, cg = (task / nrt) * CT;
+                        const int row = rt * 16 + (lane & 15), kq = lane >> 4;
+                        nn_floatx4 acc[CT];
+#pragma unroll
+                        for (int t = 0; t < CT; ++t) acc[t] = nn_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                        for (int k0 = 0; k0 < Cin; k0 += 4) {
+                            const int c = k0 + kq;
+                            float a = 0.0f;
+                            if (row < prev && c < Cin) a = (float)dx[(int64_t)row * xs + (l == 0 ? m.c0_ch[c] : c)];
+#pragma unroll
+                            for (int t = 0; t < CT; ++t) {
+                                if (cg + t < nct) {
+                                    const float b = c < Cin ? (float)wl[c * M + (cg + t) * 16 + (lane & 15)] : 0.0f;
+                                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < CT; ++t) {
+                            if (cg + t >= nct) continue;
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                const int r0 = rt * 16 + 4 * (lane >> 4) + i;
+                                if (r0 < prev) pr[(int64_t)r0 * M + (cg + t) * 16 + (lane & 15)] = (Real)acc[t][i];
+                            }
+                        }
+                    }
                 } else {
                 // the layer's matrix (used rows only) staged in LDS, 4 x 4 register tiles
                 if (!w_staged) {
