@@ -191,10 +191,10 @@ constexpr int NN_EMPTY = -1;
 constexpr int NN_RB = 4;                 // rows per thread in the projection
 
 struct NNLds {
-    size_t keys, vals, slot_site, cnt, red, misc, w, dh0, list, pr, total;
+    size_t keys, vals, slot_site, cnt, red, misc, wv, w, dh0, list, pr, total;
 };
-__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, size_t w_bytes, size_t dh0_bytes,
-                                               size_t list_bytes, size_t pr_bytes) {
+__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int win_n, size_t w_bytes,
+                                               size_t dh0_bytes, size_t list_bytes, size_t pr_bytes) {
     NNLds L;
     size_t o = 0;
     L.keys = o;      o = align16(o + (size_t)hash_cap * sizeof(int));
@@ -203,6 +203,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, size_
     L.cnt = o;       o = align16(o + (size_t)(ball_cap + 1) * sizeof(int));
     L.red = o;       o = align16(o + (NN_BS / 64 + 1) * sizeof(double));
     L.misc = o;      o = align16(o + 4 * sizeof(int));
+    L.wv = o;        o = align16(o + (size_t)win_n * 3 * sizeof(int));
     L.w = o;         o = align16(o + w_bytes);
     L.dh0 = o;       o = align16(o + dh0_bytes);
     L.list = o;      o = align16(o + list_bytes);
@@ -212,7 +213,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, size_
 }
 template <typename Real>
 __host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode) {
-    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], (size_t)m.w_elems * sizeof(Real),
+    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, (size_t)m.w_elems * sizeof(Real),
                          mode >= 1 ? (size_t)m.ball_cap[0] * m.C0 * sizeof(Real) : 0,
                          mode >= 2 ? (size_t)m.hit_elems * sizeof(int) : 0,
                          mode >= 3 ? (size_t)m.pr_elems * sizeof(Real) : 0);
@@ -311,6 +312,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int *cnt = (int *)(smem + lay.cnt);
     double *red = (double *)(smem + lay.red);
     int *misc = (int *)(smem + lay.misc);
+    // the two layer-0 windows of a step: site, hash position and species of every window slot
+    int *w_site = (int *)(smem + lay.wv), *w_pos = w_site + 2 * m.W0, *w_sp = w_pos + 2 * m.W0;
     for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
     if (tid == 0) misc[0] = 0;
     __syncthreads();
@@ -361,6 +364,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     vals[pos] = s;
                     slot_site[s] = site;
                 }
+                w_site[k] = site;
+                w_pos[k] = pos;
+                w_sp[k] = occ[site];
             }
             __syncthreads();
             counts[0] = misc[0];
@@ -369,6 +375,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             // layer-0 change of flip 1, then of flip 2 with flip 1 applied
             for (int f = 0; f < 2; ++f) {
                 const int64_t fs = f == 0 ? ia : ib;
+                const int wb = f == 0 ? 0 : m.W0;          // this flip's window in w_site / w_pos / w_sp
                 for (int e = tid; e < m.J0; e += NN_BS) {
                     const uint2 sl = m.ent_slot[e];
                     const uint32_t s4[4] = {sl.x & 0xffffu, sl.x >> 16, sl.y & 0xffffu, sl.y >> 16};
@@ -376,9 +383,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     double po = 1.0, pn = 1.0;
                     for (int q = 0; q < 4; ++q) {
                         if (s4[q] == 0xffffu) { xs[q] = -1; continue; }
-                        const int x = (int)nn_site(m, fs, m.win[s4[q]]);
-                        xs[q] = x;
-                        int sp = occ[x];
+                        const int x = w_site[wb + s4[q]];
+                        xs[q] = wb + (int)s4[q];
+                        int sp = w_sp[wb + s4[q]];
                         if (f == 1 && x == ia) sp = sj;           // flip 2 sees flip 1 applied
                         int spn = sp;
                         if (x == fs) spn = f == 0 ? sj : si;
@@ -391,7 +398,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         const Real v = (Real)(dv / m.sd[col]);
                         for (int q = 0; q < 4; ++q) {
                             if (xs[q] < 0) continue;
-                            const int s = nn_hash_find(keys, vals, mask, xs[q]);
+                            const int s = vals[w_pos[xs[q]]];
                             atomicAdd(&dh0[(int64_t)s * C0 + col], v);
                         }
                     }
