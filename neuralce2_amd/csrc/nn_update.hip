@@ -546,8 +546,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 }
                 // (2) partner sites of ball_l: claim (new sites get the next slots), then slot of every hit
                 const int nh = prev * n_part;
-                for (int i = tid; i < nh; i += NN_BS) {
-                    const int q = i / n_part, kk = i - q * n_part;
+                // hit i = (q, kk) = (i / n_part, i % n_part), stepped without divisions
+                const int q_0 = n_part > 0 ? tid / n_part : 0, kk_0 = tid - q_0 * n_part;
+                const int dq = n_part > 0 ? NN_BS / n_part : 0, dk = NN_BS - dq * n_part;
+                int q = q_0, kk = kk_0;
+                for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
+                    if (kk >= n_part) { kk -= n_part; ++q; }
                     const int site = (int)nn_site(m, slot_site[q], m.part[kk]);
                     bool claimed;
                     const int pos = nn_hash_claim(keys, mask, site, &claimed);
@@ -571,8 +575,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 __syncthreads();
                 // (3) counting sort of the hits by target: after the fill cnt[s] is the end of target s's range
                 nn_block_scan(cnt, nb, red);
-                for (int i = tid; i < nh; i += NN_BS) {
-                    const int q = i / n_part, kk = i - q * n_part;
+                q = q_0; kk = kk_0;
+                for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
+                    if (kk >= n_part) { kk -= n_part; ++q; }
                     const int pos = atomicAdd(&cnt[tgt[i]], 1);
                     list[pos] = (q << 8) | m.part_shell[kk];
                 }
@@ -591,6 +596,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 using V4 = NNv4<Real>;
                 constexpr int U2 = 2;                    // items (four channels each) per thread in flight together
                 const int D4 = D >> 2;
+                const int d4sh = (D4 & (D4 - 1)) == 0 ? __builtin_ctz((unsigned)D4) : -1;
                 for (int k0 = tid; k0 < nb * D4; k0 += U2 * NN_BS) {
                     V4 p0[U2], g0[U2];
                     int sv[U2], dv[U2];
@@ -598,7 +604,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     for (int u = 0; u < U2; ++u) {
                         const int k = k0 + u * NN_BS;
                         const int kk = k < nb * D4 ? k : 0;
-                        sv[u] = kk / D4; dv[u] = (kk - sv[u] * D4) * 4;
+                        sv[u] = d4sh >= 0 ? kk >> d4sh : kk / D4;
+                        dv[u] = (kk - sv[u] * D4) * 4;
                         const int64_t o = (int64_t)slot_site[sv[u]] * D + dv[u];
                         p0[u] = *(const V4 *)(cpre + o);
                         if (m.has_gate) g0[u] = *(const V4 *)(cgpre + o);
@@ -713,8 +720,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const int D = m.dd[l];
                     const Real *dpre = scr + m.soff_dpre[l];
                     const Real *dgpre = scr + m.soff_dgpre[l];
+                    const int dsh = (D & (D - 1)) == 0 ? __builtin_ctz((unsigned)D) : -1;
                     for (int k = tid; k < counts[l + 1] * D; k += NN_BS) {
-                        const int s = k / D, d = k - s * D;
+                        const int s = dsh >= 0 ? k >> dsh : k / D, d = k - s * D;
                         const int64_t o = (int64_t)slot_site[s] * D + d;
                         cache[m.off_pre[l] + o] += dpre[k];
                         if (m.has_gate) cache[m.off_gpre[l] + o] += dgpre[k];
