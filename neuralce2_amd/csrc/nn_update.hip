@@ -720,12 +720,32 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const int D = m.dd[l];
                     const Real *dpre = scr + m.soff_dpre[l];
                     const Real *dgpre = scr + m.soff_dgpre[l];
-                    const int dsh = (D & (D - 1)) == 0 ? __builtin_ctz((unsigned)D) : -1;
-                    for (int k = tid; k < counts[l + 1] * D; k += NN_BS) {
-                        const int s = dsh >= 0 ? k >> dsh : k / D, d = k - s * D;
-                        const int64_t o = (int64_t)slot_site[s] * D + d;
-                        cache[m.off_pre[l] + o] += dpre[k];
-                        if (m.has_gate) cache[m.off_gpre[l] + o] += dgpre[k];
+                    if (sizeof(Real) == 4 && (D & 3) == 0) {
+                        // four channels per thread: 16-byte rows of the scratch and the cache
+                        using V4 = NNv4<Real>;
+                        const int D4 = D >> 2;
+                        const int d4sh = (D4 & (D4 - 1)) == 0 ? __builtin_ctz((unsigned)D4) : -1;
+                        for (int g = 0; g < (m.has_gate ? 2 : 1); ++g) {
+                            Real *cbase = cache + (g == 0 ? m.off_pre[l] : m.off_gpre[l]);
+                            const Real *dbase = g == 0 ? dpre : dgpre;
+                            for (int k = tid; k < counts[l + 1] * D4; k += NN_BS) {
+                                const int s = d4sh >= 0 ? k >> d4sh : k / D4, d = (k - s * D4) * 4;
+                                V4 *cp = (V4 *)(cbase + (int64_t)slot_site[s] * D + d);
+                                const V4 dv = *(const V4 *)(dbase + 4 * (int64_t)k);
+                                V4 c = *cp;
+#pragma unroll
+                                for (int e = 0; e < 4; ++e) c.v[e] += dv.v[e];
+                                *cp = c;
+                            }
+                        }
+                    } else {
+                        const int dsh = (D & (D - 1)) == 0 ? __builtin_ctz((unsigned)D) : -1;
+                        for (int k = tid; k < counts[l + 1] * D; k += NN_BS) {
+                            const int s = dsh >= 0 ? k >> dsh : k / D, d = k - s * D;
+                            const int64_t o = (int64_t)slot_site[s] * D + d;
+                            cache[m.off_pre[l] + o] += dpre[k];
+                            if (m.has_gate) cache[m.off_gpre[l] + o] += dgpre[k];
+                        }
                     }
                 }
                 if (tid == 0) {
