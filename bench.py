@@ -132,6 +132,28 @@ def served_bench(args):
     return res, st, base, specs
 
 
+def batched_bench(args):
+    """Batched uneven jobs: every request present at t = 0, fcc n^3 cubic cells (n in 4..32, log-uniform in sites),
+    x in 0.1..0.9, 16-64 replicas, CE and NeuralCE jobs 3:1 (the NeuralCE jobs on the 3-layer SiLU network), all
+    run together by the continuous-batching server: one chunk launch for every CE job, one for every NeuralCE job."""
+    from neuralce2_amd.serve import Server, job_stream
+    from neuralce2_amd.synthetic import fcc_cluster_space, fcc_cell
+    from neuralce2_amd.nce.mc_model import NNModel, silu_network
+    cs = fcc_cluster_space((9.0, 6.0, 5.0))
+    P = silu_network(len(cs), 64, 3, 1, seed=args.seed)
+    mu, sd = np.zeros(len(cs)), np.ones(len(cs))
+
+    def nce_model(n):
+        return NNModel(cs, fcc_cell(n, 0.5, 0), P, mu, sd, [0.0, 3.3], act="silu", readout="sum")
+
+    specs = job_stream(args.jobs, 0.0, seed=args.seed, n_min=4, n_max=args.n_max, t_min=16, t_max=64,
+                       n_steps=args.job_steps, chunk=args.chunk, nce_every=4)
+    srv = Server(chunk=args.chunk, precision=args.precision or "mixed", nn_precision="bf16", nce_model=nce_model)
+    srv.run([s for s in specs if s.kind == "ce"][:1] + [s for s in specs if s.kind == "nce"][:1])   # warm-up
+    res, st = srv.run(specs)
+    return res, st, specs
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +167,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--served", action="store_true", help="served request stream (p50/p99 job latency)")
+    ap.add_argument("--batched", action="store_true",
+                    help="uneven CE + NeuralCE jobs (3:1), all present at once (use --jobs 512 --job-steps N)")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=100.0, help="requests per second (Poisson)")
     ap.add_argument("--job-steps", type=int, default=100000)
@@ -195,6 +219,22 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.batched:
+        res, st, specs = batched_bench(args)
+        n_ce = sum(1 for s in specs if s.kind == "ce")
+        line = {"metric": "MC swap attempts/s (all replicas, batched uneven jobs)", "value": st["attempts_per_s"],
+                "unit": "attempts/s", "n_gpus": 1, "steps": st["n_ticks"], "warmup": 1,
+                "ms_per_step": st["wall_s"] / max(st["n_ticks"], 1) * 1e3, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32 dE / bf16 mfma NeuralCE / f64 totals",
+                "data": "synthetic batch (seeded)",
+                "config": {"workload": f"{len(specs)} PT jobs at once ({n_ce} CE, {len(specs) - n_ce} NeuralCE 3-layer "
+                                       f"SiLU), fcc n^3 cubic cells log-uniform 256-{4 * args.n_max ** 3} sites, "
+                                       f"x in 0.1..0.9, 16-64 replicas, {specs[0].n_steps} steps each, chunk "
+                                       f"{args.chunk}"},
+                "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"], "wall_s": st["wall_s"],
+                "attempts": st["attempts"], "roofline": None, "cpu_baseline": None}
+        print(json.dumps(line), flush=True)
         return
     if args.served:
         res, st, base, specs = served_bench(args)
