@@ -266,14 +266,36 @@ class Server:
         attempts = 0
         t0 = time.perf_counter()
         n_ticks = 0
+        nn_inflight = None                  # event after the NeuralCE chunk in flight on its stream
+
+        def retire(group, stream):
+            nonlocal mem_used
+            done = [j for j in group if j.steps_done >= j.spec.n_steps]
+            for j in done:
+                with torch.cuda.stream(stream):
+                    E, cv, acc, occ = j.result(self.keep_occ)
+                fin = time.perf_counter() - t0
+                results.append(JobResult(j.spec.job_id, j.spec.kind, j.N, j.temps.astype(np.float64), E, cv,
+                                         j.steps_done * j.state.R, acc, j.spec.arrival, j.admitted, fin, occ))
+                mem_used -= j.mem
+                group.remove(j)
+                if log:
+                    log(f"job {j.spec.job_id} ({j.spec.kind}, {j.N} sites, {j.state.R} T) done at {fin:.2f} s, "
+                        f"latency {fin - j.spec.arrival:.2f} s")
+
         while pending or active_ce or active_nn:
             now = time.perf_counter() - t0
-            # admission in arrival order while the memory fits
+            # admission in arrival order while the memory fits; a NeuralCE job joins at the next NeuralCE launch
             while pending and pending[0].arrival <= now and mem_used + self._bytes(pending[0]) <= self.mem_budget:
-                spec = pending.pop(0)
-                with torch.cuda.stream(self.stream_ce if spec.kind == "ce" else self.stream_nn):
+                spec = pending[0]
+                if spec.kind != "ce" and nn_inflight is not None:
+                    break
+                pending.pop(0)
+                stream = self.stream_ce if spec.kind == "ce" else self.stream_nn
+                with torch.cuda.stream(stream):
                     job = (_CEJob(spec, self._gce(spec.n), self.chunk) if spec.kind == "ce"
                            else _NCEJob(spec, self._nn(spec.n), self.nn_chunk, self.nn_precision))
+                stream.synchronize()
                 job.admitted = time.perf_counter() - t0
                 job.mem = self._bytes(spec)
                 mem_used += job.mem
@@ -282,46 +304,41 @@ class Server:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - (time.perf_counter() - t0))))
                 continue
-            torch.cuda.synchronize(self.device)
-            # one tick: every CE job in one launch, every NeuralCE job in one launch on the second stream
+            # the NeuralCE jobs: one launch on their stream, left running while the CE jobs tick
+            if nn_inflight is None and active_nn:
+                with torch.cuda.stream(self.stream_nn):
+                    self.ctx.nn_run_chunk([j.state.job() for j in active_nn], self.nn_chunk, self.nn_precision)
+                    nn_inflight = torch.cuda.Event()
+                    nn_inflight.record(self.stream_nn)
+            # one CE tick: every CE job in one launch, then the moments, the exchange and the retirements
             if active_ce:
                 tile = self._tile_for(active_ce)
                 with torch.cuda.stream(self.stream_ce):
                     self.ctx.pt_run_chunk([j.state.job(accumulate=True) for j in active_ce], self.chunk,
                                           self.precision, tile)
-            if active_nn:
-                with torch.cuda.stream(self.stream_nn):
-                    self.ctx.nn_run_chunk([j.state.job() for j in active_nn], self.nn_chunk, self.nn_precision)
-            torch.cuda.synchronize(self.device)
-            n_ticks += 1
-            if active_ce:
-                for j in active_ce:
-                    j.state.merge_accumulators(self.chunk, j.state.slot_of_rep.cpu().numpy())
-                    j.steps_done += self.chunk
-                    attempts += self.chunk * j.state.R
-                with torch.cuda.stream(self.stream_ce):
+                    for j in active_ce:
+                        j.state.merge_accumulators(self.chunk, j.state.slot_of_rep.cpu().numpy())
+                        j.steps_done += self.chunk
+                        attempts += self.chunk * j.state.R
                     self.ctx.pt_exchange([j.state.exchange_job(j.n_ex + 1, reorder_last=True) for j in active_ce])
+                self.stream_ce.synchronize()
                 for j in active_ce:
                     j.n_ex += 1
-            for j in active_nn:
+                n_ticks += 1
+                retire(active_ce, self.stream_ce)
+            elif nn_inflight is not None:
+                nn_inflight.synchronize()           # nothing else to run meanwhile
+            # the NeuralCE launch, once finished: exchange, counters and retirements
+            if nn_inflight is not None and nn_inflight.query():
+                nn_inflight = None
                 with torch.cuda.stream(self.stream_nn):
-                    j.after_chunk(self.nn_chunk)
-                j.steps_done += self.nn_chunk
-                attempts += self.nn_chunk * j.state.R
-            torch.cuda.synchronize(self.device)
-            # retirement
-            for group in (active_ce, active_nn):
-                done = [j for j in group if j.steps_done >= j.spec.n_steps]
-                for j in done:
-                    E, cv, acc, occ = j.result(self.keep_occ)
-                    fin = time.perf_counter() - t0
-                    results.append(JobResult(j.spec.job_id, j.spec.kind, j.N, j.temps.astype(np.float64), E, cv,
-                                             j.steps_done * j.state.R, acc, j.spec.arrival, j.admitted, fin, occ))
-                    mem_used -= j.mem
-                    group.remove(j)
-                    if log:
-                        log(f"job {j.spec.job_id} ({j.spec.kind}, {j.N} sites, {j.state.R} T) done at {fin:.2f} s, "
-                            f"latency {fin - j.spec.arrival:.2f} s")
+                    for j in active_nn:
+                        j.after_chunk(self.nn_chunk)
+                        j.steps_done += self.nn_chunk
+                        attempts += self.nn_chunk * j.state.R
+                self.stream_nn.synchronize()
+                n_ticks += 1
+                retire(active_nn, self.stream_nn)
         wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in results])
         stats = dict(wall_s=wall, attempts=int(attempts), attempts_per_s=attempts / wall, n_jobs=len(results),
