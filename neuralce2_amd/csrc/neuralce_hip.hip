@@ -563,19 +563,21 @@ __host__ __device__ inline size_t fs_bytes(int nlo, int nfull, int ntail) {
                          (size_t)ntail * sizeof(FsTail));
 }
 
-// |c| (sign bit clear) with the low bit of n as its sign
 // (a & m) ^ t in one v_bitop3 (truth table over S0 = 0xF0, S1 = 0xCC, S2 = 0xAA)
-__device__ inline uint32_t and_xor(uint32_t a, uint32_t m, uint32_t t) { return __builtin_amdgcn_bitop3_b32(a, m, t, 0x6A); }
+__device__ inline uint32_t and_xor(uint32_t a, uint32_t m, uint32_t t) {
+    return __builtin_amdgcn_bitop3_b32(a, m, t, 0x6A);
+}
+// |c| (sign bit clear) with the low bit of n as its sign
 __device__ inline float fs_term(uint32_t c, uint32_t n) { return __int_as_float((int)((n << 31) | c)); }
 __device__ inline double fs_term(uint64_t c, uint32_t n) {
     return __longlong_as_double((long long)(((uint64_t)(n << 31) << 32) | c));
 }
 
-// Speculative parallel steps: wave w of a W-wave workgroup evaluates step t0 + w of the same replica from the
-// current occupations; the window loads of all W steps are in flight together.  A uniform pass then applies the
-// steps in order (Metropolis, running total, statistics); a step whose windows contain a site changed by an
-// earlier accepted step of the same group is re-evaluated on patched species before its decision, so the result is
-// exactly the sequential chain.
+// Speculative parallel steps: a group is the run of steps holding the next W different-species swaps of one
+// replica; the wave of rank k evaluates the group's k-th swap from the current occupations, so the window loads of
+// all W swaps are in flight together.  The decisions are then taken in step order (Metropolis, running total,
+// statistics); a swap whose windows contain a site changed by an earlier accepted swap of the same group is
+// re-evaluated on patched species before its decision, so the result is exactly the sequential chain.
 struct SpecRec {
     double dE;
     double lu;                       // log of the step's Metropolis uniform, in the precision of the decision
@@ -587,7 +589,7 @@ struct SpecLds {
     size_t slot, coef, val, prop, rec, commit, fs, pp, meta, total;
 };
 
-constexpr int SPEC_KP = 64;            // proposals scanned per group (one lane of wave 0 each)
+constexpr int SPEC_KP = 64;            // proposals scanned per group (one lane of wave 0 each); W of them kept
 
 struct SpecProp {
     int ia, ib;
@@ -605,7 +607,7 @@ __host__ __device__ inline SpecLds<Real> spec_lds_layout(int J, int Wcap, int wa
     L.rec = o;    o = align16(o + (size_t)waves * sizeof(SpecRec));
     L.commit = o; o = align16(o + (size_t)waves * sizeof(int4));
     L.fs = o;     o = align16(o + fs_bytes<Real>(nlo, nfull, ntail));
-    L.pp = o;     o = align16(o + (size_t)SPEC_KP * sizeof(SpecProp));
+    L.pp = o;     o = align16(o + (size_t)waves * sizeof(SpecProp));
     L.meta = o;   o = align16(o + 2 * sizeof(unsigned long long));
     L.total = o;
     return L;
