@@ -130,9 +130,11 @@ int nce_ce_delta(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_repli
  * Replaces run_chunk = lax.scan(micro) of vmapped delta mc_step (neuralce/ce/pt_mcmc_ce.py:554-566,765-818) with
  * make_fast_swap_implicit proposals (neuralce/ce/ce_gpu_implicit.py:194-211).
  * jobs: HOST array of n_jobs job descriptors (copied to the device stream-ordered).
- * tile: 0 = default.  Binary one-basis models run the speculative kernel: tile 4, 8 or 16 = steps of a replica
- * evaluated in parallel (one wave each; 16 falls back to 8 where the tables do not fit in LDS); other models run
- * the general kernel: tile 1, 2, 4 or 8 = waves per step. */
+ * tile: 0 = default.  Binary one-basis models run the speculative kernel: tile 4, 8 or 16 = different-species
+ * swaps of a replica evaluated in parallel (one wave each; 16 falls back to 8 where the tables do not fit in LDS).
+ * The default takes 16, 8 or 4 by replicas per CU (< 2, < 4, more), then halves it while the expected number of
+ * overlapping window pairs in a group, W(W-1)/2 * 4 * window / sites of the smallest cell, exceeds 2.  Other
+ * models run the general kernel: tile 1, 2, 4 or 8 = waves per step. */
 int nce_pt_run_chunk(nce_ctx *ctx, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps, int32_t precision,
                      int32_t tile, void *stream);
 

@@ -1681,7 +1681,18 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
     const int per_cu = (int)((items.size() + c->n_cu - 1) / c->n_cu);
     // general kernel: waves cooperating on one step of a replica; speculative kernel: steps evaluated in parallel
     int W = tile == 0 ? (per_cu >= 4 ? 1 : per_cu >= 2 ? 2 : 4) : (tile > 8 ? 8 : tile);
-    const int tile_spec = (tile == 4 || tile == 8 || tile == 16) ? tile : (per_cu >= 4 ? 4 : per_cu >= 2 ? 8 : 16);
+    // default speculative tile: fewer waves when several replicas share a CU, and when the windows of a group's
+    // swaps would overlap often (expected pairs W(W-1)/2 * 4 * window / N of the smallest cell above 2: each
+    // overlap with an accepted earlier swap costs a re-evaluation)
+    int tile_spec = (tile == 4 || tile == 8 || tile == 16) ? tile : (per_cu >= 4 ? 4 : per_cu >= 2 ? 8 : 16);
+    if (tile == 0) {
+        double dens = 0.0;                             // largest window / sites over the jobs
+        for (int j = 0; j < n_jobs; ++j) {
+            const ModelDev &m = c->models[jobs[j].model];
+            dens = std::max(dens, (double)m.Wmax / (double)std::max<int64_t>(m.n_sites, 1));
+        }
+        while (tile_spec > 4 && 0.5 * tile_spec * (tile_spec - 1) * 4.0 * dens > 2.0) tile_spec /= 2;
+    }
     // job table and work items in one staged buffer
     const size_t jb_bytes = align16(sizeof(nce_pt_job) * n_jobs);
     std::vector<unsigned char> buf(jb_bytes + items.size() * sizeof(int2));
