@@ -245,7 +245,9 @@ class Server:
         import torch
         n_items = sum(j.state.R for j in ce_jobs)
         n_cu = torch.cuda.get_device_properties(self.device).multi_processor_count
-        key = (min(4, (n_items + n_cu - 1) // n_cu), self.precision)
+        # replicas per CU, precision and the smallest cell (window overlaps within a group grow as the cell shrinks)
+        key = (min(4, (n_items + n_cu - 1) // n_cu), self.precision,
+               int(math.log2(max(1, min(j.N for j in ce_jobs)))))
         if key not in self._tile_cache:
             self._tile_cache[key] = autotune_tile(self.ctx, [j.state for j in ce_jobs], self.precision)
         return self._tile_cache[key]
