@@ -191,9 +191,9 @@ constexpr int NN_EMPTY = -1;
 constexpr int NN_RB = 4;                 // rows per thread in the projection
 
 struct NNLds {
-    size_t keys, vals, slot_site, cnt, red, misc, wv, w, dh0, list, pr, total;
+    size_t keys, vals, slot_site, cnt, red, misc, wv, part, w, dh0, list, pr, total;
 };
-__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int win_n, size_t w_bytes,
+__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int win_n, int n_part, size_t w_bytes,
                                                size_t dh0_bytes, size_t list_bytes, size_t pr_bytes) {
     NNLds L;
     size_t o = 0;
@@ -204,6 +204,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
     L.red = o;       o = align16(o + (NN_BS / 64 + 1) * sizeof(double));
     L.misc = o;      o = align16(o + 4 * sizeof(int));
     L.wv = o;        o = align16(o + (size_t)win_n * 3 * sizeof(int));
+    L.part = o;      o = align16(o + (size_t)n_part * (sizeof(int4) + sizeof(int)));
     L.w = o;         o = align16(o + w_bytes);
     L.dh0 = o;       o = align16(o + dh0_bytes);
     L.list = o;      o = align16(o + list_bytes);
@@ -213,7 +214,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
 }
 template <typename Real>
 __host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode) {
-    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, (size_t)m.w_elems * sizeof(Real),
+    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, m.n_part, (size_t)m.w_elems * sizeof(Real),
                          mode >= 1 ? (size_t)m.ball_cap[0] * m.C0 * sizeof(Real) : 0,
                          mode >= 2 ? (size_t)m.hit_elems * sizeof(int) : 0,
                          mode >= 3 ? (size_t)m.pr_elems * sizeof(Real) : 0);
@@ -314,6 +315,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int *misc = (int *)(smem + lay.misc);
     // the two layer-0 windows of a step: site, hash position and species of every window slot
     int *w_site = (int *)(smem + lay.wv), *w_pos = w_site + 2 * m.W0, *w_sp = w_pos + 2 * m.W0;
+    // shell partners (offset and shell of each), staged once per launch
+    int4 *s_part = (int4 *)(smem + lay.part);
+    int *s_part_shell = (int *)(s_part + m.n_part);
+    for (int k = tid; k < m.n_part; k += NN_BS) {
+        s_part[k] = m.part[k];
+        s_part_shell[k] = m.part_shell[k];
+    }
     for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
     if (tid == 0) misc[0] = 0;
     __syncthreads();
@@ -552,7 +560,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 int q = q_0, kk = kk_0;
                 for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
                     if (kk >= n_part) { kk -= n_part; ++q; }
-                    const int site = (int)nn_site(m, slot_site[q], m.part[kk]);
+                    const int site = (int)nn_site(m, slot_site[q], s_part[kk]);
                     bool claimed;
                     const int pos = nn_hash_claim(keys, mask, site, &claimed);
                     if (claimed) {
@@ -579,7 +587,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
                     if (kk >= n_part) { kk -= n_part; ++q; }
                     const int pos = atomicAdd(&cnt[tgt[i]], 1);
-                    list[pos] = (q << 8) | m.part_shell[kk];
+                    list[pos] = (q << 8) | s_part_shell[kk];
                 }
                 __syncthreads();
                 // (4) change of the pre-activations, new activations, change of the layer output
