@@ -220,7 +220,12 @@ __host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode) {
                          mode >= 3 ? (size_t)m.pr_elems * sizeof(Real) : 0);
 }
 
-__device__ inline uint32_t nn_hash(int site, int mask) { return ((uint32_t)site * 2654435761u) & (uint32_t)mask; }
+// multiplicative hashing: the position is the top log2(cap) bits of the 32-bit product (cap = mask + 1 >= 64, a power
+// of two). The low bits would depend on site mod cap alone, so sites a multiple of cap apart (one cell layer along the
+// slowest axis, for instance) would always share a position and lengthen the probe chains.
+__device__ inline uint32_t nn_hash(int site, int mask) {
+    return (((uint32_t)site * 2654435761u) >> __clz(mask)) & (uint32_t)mask;
+}
 
 // phase 1 of an insertion: claim or find the key; returns the table position and whether this thread claimed it
 __device__ inline int nn_hash_claim(int *keys, int mask, int site, bool *claimed) {
