@@ -193,7 +193,7 @@ constexpr int NN_RB = 4;                 // rows per thread in the projection
 struct NNLds {
     size_t keys, vals, slot_site, cnt, red, misc, wv, part, w, dh0, list, pr, total;
 };
-__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int win_n, int n_part, size_t w_bytes,
+__host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int win_n, int n_part, int n_c0, size_t w_bytes,
                                                size_t dh0_bytes, size_t list_bytes, size_t pr_bytes) {
     NNLds L;
     size_t o = 0;
@@ -204,7 +204,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
     L.red = o;       o = align16(o + (NN_BS / 64 + 1) * sizeof(double));
     L.misc = o;      o = align16(o + 4 * sizeof(int));
     L.wv = o;        o = align16(o + (size_t)win_n * 3 * sizeof(int));
-    L.part = o;      o = align16(o + (size_t)n_part * (sizeof(int4) + sizeof(int)));
+    L.part = o;      o = align16(o + (size_t)n_part * (sizeof(int4) + sizeof(int)) + (size_t)n_c0 * sizeof(int));
     L.w = o;         o = align16(o + w_bytes);
     L.dh0 = o;       o = align16(o + dh0_bytes);
     L.list = o;      o = align16(o + list_bytes);
@@ -214,7 +214,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
 }
 template <typename Real>
 __host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode) {
-    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, m.n_part, (size_t)m.w_elems * sizeof(Real),
+    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, m.n_part, m.c0_eff, (size_t)m.w_elems * sizeof(Real),
                          mode >= 1 ? (size_t)m.ball_cap[0] * m.C0 * sizeof(Real) : 0,
                          mode >= 2 ? (size_t)m.hit_elems * sizeof(int) : 0,
                          mode >= 3 ? (size_t)m.pr_elems * sizeof(Real) : 0);
@@ -295,6 +295,24 @@ template <typename Real>
 struct alignas(16) NNv4 {
     Real v[4];
 };
+// four channels at p, moved as one vector access in p's address space (LDS or global)
+template <typename Real>
+__device__ inline NNv4<Real> nn_ld4(const Real *p) {
+    typedef Real vec __attribute__((ext_vector_type(4)));
+    const vec x = *(const vec *)p;
+    return NNv4<Real>{{x[0], x[1], x[2], x[3]}};
+}
+template <typename Real>
+__device__ inline NNv4<Real> nn_ld4(const NCE_GLOBAL Real *p) {
+    typedef Real vec __attribute__((ext_vector_type(4)));
+    const vec x = *(const NCE_GLOBAL vec *)p;
+    return NNv4<Real>{{x[0], x[1], x[2], x[3]}};
+}
+template <typename Real>
+__device__ inline void nn_st4(NCE_GLOBAL Real *p, const NNv4<Real> &v) {
+    typedef Real vec __attribute__((ext_vector_type(4)));
+    *(NCE_GLOBAL vec *)p = vec{v.v[0], v.v[1], v.v[2], v.v[3]};
+}
 typedef __bf16 nn_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float nn_floatx4 __attribute__((ext_vector_type(4)));
 
@@ -320,28 +338,40 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int *misc = (int *)(smem + lay.misc);
     // the two layer-0 windows of a step: site, hash position and species of every window slot
     int *w_site = (int *)(smem + lay.wv), *w_pos = w_site + 2 * m.W0, *w_sp = w_pos + 2 * m.W0;
-    // shell partners (offset and shell of each), staged once per launch
+    // shell partners (offset and shell of each) and the used layer-0 channels, staged once per launch
     int4 *s_part = (int4 *)(smem + lay.part);
     int *s_part_shell = (int *)(s_part + m.n_part);
+    int *s_c0 = s_part_shell + m.n_part;
     for (int k = tid; k < m.n_part; k += NN_BS) {
         s_part[k] = m.part[k];
         s_part_shell[k] = m.part_shell[k];
     }
+    for (int k = tid; k < m.c0_eff; k += NN_BS) s_c0[k] = m.c0_ch[k];
     for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
     if (tid == 0) misc[0] = 0;
     __syncthreads();
 
-    int8_t *occ = job.occ + (int64_t)r * job.n_sites;
-    Real *cache = (Real *)job.cache + (int64_t)r * m.cache_elems;
-    Real *scr = (Real *)job.scratch + (int64_t)r * m.scratch_elems;
+    // buffers in device memory are addressed as global (flat accesses would also count against lgkmcnt, so every
+    // LDS wait would drain the cache and scratch traffic in flight)
+    NCE_GLOBAL int8_t *occ = gptr(job.occ) + (int64_t)r * job.n_sites;
+    NCE_GLOBAL Real *cache = gptr((Real *)job.cache) + (int64_t)r * m.cache_elems;
+    NCE_GLOBAL Real *scr = gptr((Real *)job.scratch) + (int64_t)r * m.scratch_elems;
     Real *wl = (Real *)(smem + lay.w);                 // projection matrix of the current layer
-    Real *dh0 = MODE >= 1 ? (Real *)(smem + lay.dh0) : scr + m.soff_dh0;
-    int *list = MODE >= 2 ? (int *)(smem + lay.list) : (int *)(scr + m.soff_list);
-    Real *pr = MODE >= 3 ? (Real *)(smem + lay.pr) : scr + m.soff_pr;
-    int *tgt = (int *)(scr + m.soff_tgt);
-    const Real *W = nn_w<Real>(m);
-    const Real *lin = nn_lin<Real>(m);
-    const Real *hw = nn_headw<Real>(m);
+    // per-step arrays: in LDS where MODE places them, else in the replica's scratch
+    std::conditional_t<(MODE >= 1), Real *, NCE_GLOBAL Real *> dh0;
+    std::conditional_t<(MODE >= 2), int *, NCE_GLOBAL int *> list;
+    std::conditional_t<(MODE >= 3), Real *, NCE_GLOBAL Real *> pr;
+    if constexpr (MODE >= 1) dh0 = (Real *)(smem + lay.dh0); else dh0 = scr + m.soff_dh0;
+    if constexpr (MODE >= 2) list = (int *)(smem + lay.list); else list = (NCE_GLOBAL int *)(scr + m.soff_list);
+    if constexpr (MODE >= 3) pr = (Real *)(smem + lay.pr); else pr = scr + m.soff_pr;
+    NCE_GLOBAL int *tgt = (NCE_GLOBAL int *)(scr + m.soff_tgt);
+    const NCE_GLOBAL Real *W = gptr(nn_w<Real>(m));
+    const NCE_GLOBAL Real *lin = gptr(nn_lin<Real>(m));
+    const NCE_GLOBAL Real *hw = gptr(nn_headw<Real>(m));
+    const NCE_GLOBAL int *win = (const NCE_GLOBAL int *)gptr(m.win);
+    const NCE_GLOBAL uint64_t *ent_slot = (const NCE_GLOBAL uint64_t *)gptr(m.ent_slot);
+    const NCE_GLOBAL int *ent_col = gptr(m.ent_col);
+    const NCE_GLOBAL double *ent_coef = gptr(m.ent_coef), *sd = gptr(m.sd);
     double E = job.energy[r];
     const double beta = job.beta[r];
     const uint64_t step0 = (uint64_t)job.step[r];
@@ -354,7 +384,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
         int64_t ia, ib;
         double u;
         if (job.inj_ia) {
-            ia = job.inj_ia[inj_base + t]; ib = job.inj_ib[inj_base + t]; u = job.inj_u[inj_base + t];
+            ia = gptr(job.inj_ia)[inj_base + t]; ib = gptr(job.inj_ib)[inj_base + t]; u = gptr(job.inj_u)[inj_base + t];
         } else {
             const Proposal p = draw_proposal_1b(job.seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)m.n_cells);
             ia = p.ia; ib = p.ib; u = p.u;
@@ -369,7 +399,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
         } else {
             // ball 0: the two layer-0 windows
             for (int k = tid; k < 2 * m.W0; k += NN_BS) {
-                const int site = (int)nn_site(m, k < m.W0 ? ia : ib, m.win[k < m.W0 ? k : k - m.W0]);
+                const NCE_GLOBAL int *wk = win + 4 * (k < m.W0 ? k : k - m.W0);
+                const int site = (int)nn_site(m, k < m.W0 ? ia : ib, make_int4(0, wk[1], wk[2], wk[3]));
                 bool claimed;
                 const int pos = nn_hash_claim(keys, mask, site, &claimed);
                 if (claimed) {
@@ -390,8 +421,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const int64_t fs = f == 0 ? ia : ib;
                 const int wb = f == 0 ? 0 : m.W0;          // this flip's window in w_site / w_pos / w_sp
                 for (int e = tid; e < m.J0; e += NN_BS) {
-                    const uint2 sl = m.ent_slot[e];
-                    const uint32_t s4[4] = {sl.x & 0xffffu, sl.x >> 16, sl.y & 0xffffu, sl.y >> 16};
+                    const uint64_t sl = ent_slot[e];
+                    const uint32_t lo = (uint32_t)sl, hi = (uint32_t)(sl >> 32);
+                    const uint32_t s4[4] = {lo & 0xffffu, lo >> 16, hi & 0xffffu, hi >> 16};
                     int xs[4];
                     double po = 1.0, pn = 1.0;
                     for (int q = 0; q < 4; ++q) {
@@ -405,14 +437,14 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         po *= sp ? 1.0 : -1.0;
                         pn *= spn ? 1.0 : -1.0;
                     }
-                    const double dv = (pn - po) * m.ent_coef[e];
+                    const double dv = (pn - po) * ent_coef[e];
                     if (dv != 0.0) {
-                        const int col = m.ent_col[e];
-                        const Real v = (Real)(dv / m.sd[col]);
+                        const int col = ent_col[e];
+                        const Real v = (Real)(dv / sd[col]);
                         for (int q = 0; q < 4; ++q) {
                             if (xs[q] < 0) continue;
                             const int s = vals[w_pos[xs[q]]];
-                            atomicAdd(&dh0[(int64_t)s * C0 + col], v);
+                            atomicAdd((Real *)&dh0[(int64_t)s * C0 + col], v);
                         }
                     }
                 }
@@ -424,9 +456,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             for (int l = 0; l < L; ++l) {
                 const int Cin = l == 0 ? m.c0_eff : m.cin[l], D = m.dd[l], M = m.mcat[l];
                 const int prev = counts[l];
-                const Real *dx = l == 0 ? dh0 : scr + m.soff_dh[(l - 1) & 1];
-                const int xs = l == 0 ? C0 : m.cin[l];
-                const Real *Wc = W + m.off_wcat[l];
+                // the previous layer's change: layer 0 in dh0 (used channels only), deeper layers in the scratch
+                const NCE_GLOBAL Real *dxg = scr + m.soff_dh[(l - 1) & 1];
+                auto dx = [&](int64_t row, int c) -> Real {
+                    return l == 0 ? (Real)dh0[row * C0 + s_c0[c]] : dxg[row * m.cin[l] + c];
+                };
+                const NCE_GLOBAL Real *Wc = W + m.off_wcat[l];
                 // (1) pr[q][j] = dx[q] . Wcat[:, j]
                 if constexpr (MFMA) {
                     // BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation): the layer's matrix staged in LDS as
@@ -437,7 +472,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     if (!w_staged) {
                         for (int k = tid; k < M * Kp; k += NN_BS) {
                             const int j = k / Kp, c = k - j * Kp;
-                            wt[k] = c < Cin ? (__bf16)Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j] : (__bf16)0.0f;
+                            wt[k] = c < Cin ? (__bf16)Wc[(int64_t)(l == 0 ? s_c0[c] : c) * M + j] : (__bf16)0.0f;
                         }
                         __syncthreads();
                     }
@@ -456,7 +491,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             for (int i = 0; i < 8; ++i) {
                                 const int c = k0 + kb + i;
                                 float x = 0.0f;
-                                if (row < prev && c < Cin) x = (float)dx[(int64_t)row * xs + (l == 0 ? m.c0_ch[c] : c)];
+                                if (row < prev && c < Cin) x = (float)dx(row, c);
                                 av[i] = (__bf16)x;
                             }
 #pragma unroll
@@ -485,7 +520,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     if (!w_staged) {
                         for (int k = tid; k < Cin * M; k += NN_BS) {
                             const int c = k / M, j = k - c * M;
-                            wl[k] = Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j];
+                            wl[k] = Wc[(int64_t)(l == 0 ? s_c0[c] : c) * M + j];
                         }
                         __syncthreads();
                     }
@@ -501,7 +536,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         for (int k0 = 0; k0 < Cin; k0 += 4) {
                             const int c = k0 + kq;
                             float a = 0.0f;
-                            if (row < prev && c < Cin) a = (float)dx[(int64_t)row * xs + (l == 0 ? m.c0_ch[c] : c)];
+                            if (row < prev && c < Cin) a = (float)dx(row, c);
 #pragma unroll
                             for (int t = 0; t < CT; ++t) {
                                 if (cg + t < nct) {
@@ -525,7 +560,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 if (!w_staged) {
                     for (int k = tid; k < Cin * M; k += NN_BS) {
                         const int c = k / M, j = k - c * M;
-                        wl[k] = Wc[(int64_t)(l == 0 ? m.c0_ch[c] : c) * M + j];
+                        wl[k] = Wc[(int64_t)(l == 0 ? s_c0[c] : c) * M + j];
                     }
                     __syncthreads();
                 }
@@ -539,12 +574,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
 #pragma unroll
                         for (int u = 0; u < 4; ++u) acc[i][u] = (Real)0;
                     for (int c = 0; c < Cin; ++c) {
-                        const int cx = l == 0 ? m.c0_ch[c] : c;
                         Real wv[4], xv[NN_RB];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) wv[u] = j0 + u < M ? wl[c * M + j0 + u] : (Real)0;
 #pragma unroll
-                        for (int i = 0; i < NN_RB; ++i) xv[i] = q0 + i < prev ? dx[(int64_t)(q0 + i) * xs + cx] : (Real)0;
+                        for (int i = 0; i < NN_RB; ++i) xv[i] = q0 + i < prev ? dx(q0 + i, c) : (Real)0;
 #pragma unroll
                         for (int i = 0; i < NN_RB; ++i)
 #pragma unroll
@@ -596,11 +630,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 }
                 __syncthreads();
                 // (4) change of the pre-activations, new activations, change of the layer output
-                Real *dpre = scr + m.soff_dpre[l];
-                Real *dgpre = scr + m.soff_dgpre[l];
-                Real *dh = scr + m.soff_dh[l & 1];
-                const Real *cpre = cache + m.off_pre[l];
-                const Real *cgpre = cache + m.off_gpre[l];
+                NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
+                NCE_GLOBAL Real *dgpre = scr + m.soff_dgpre[l];
+                NCE_GLOBAL Real *dh = scr + m.soff_dh[l & 1];
+                const NCE_GLOBAL Real *cpre = cache + m.off_pre[l];
+                const NCE_GLOBAL Real *cgpre = cache + m.off_gpre[l];
                 const bool last = l + 1 == L;
                 const int og = D * (1 + P);
                 constexpr int U = 4;                     // items per thread in flight together
@@ -620,8 +654,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         sv[u] = d4sh >= 0 ? kk >> d4sh : kk / D4;
                         dv[u] = (kk - sv[u] * D4) * 4;
                         const int64_t o = (int64_t)slot_site[sv[u]] * D + dv[u];
-                        p0[u] = *(const V4 *)(cpre + o);
-                        if (m.has_gate) g0[u] = *(const V4 *)(cgpre + o);
+                        p0[u] = nn_ld4<Real>(cpre + o);
+                        if (m.has_gate) g0[u] = nn_ld4<Real>(cgpre + o);
                         else g0[u] = V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
                     }
 #pragma unroll
@@ -629,11 +663,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         const int k = k0 + u * NN_BS;
                         if (k >= nb * D4) break;
                         const int s = sv[u], d = dv[u];
-                        V4 dp = s < prev ? *(const V4 *)(pr + (int64_t)s * M + d) : V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
+                        V4 dp = s < prev ? nn_ld4<Real>(pr + (int64_t)s * M + d) : V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
                         V4 dg = V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
                         const int j0 = s == 0 ? 0 : cnt[s - 1], j1 = cnt[s];
                         auto at = [&](int h) -> int64_t { return (int64_t)(h >> 8) * M + D * (h & 0xff) + d; };
-                        auto rd = [&](int64_t a) -> V4 { return *(const V4 *)(pr + a); };
+                        auto rd = [&](int64_t a) -> V4 { return nn_ld4<Real>(pr + a); };
                         int j = j0;
                         for (; j + 4 <= j1; j += 4) {
                             const int64_t a0 = at(list[j]), a1 = at(list[j + 1]), a2 = at(list[j + 2]),
@@ -663,13 +697,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         for (int e = 0; e < 4; ++e)
                             dhv.v[e] = nn_act<Real>(m, p0[u].v[e] + dp.v[e], g0[u].v[e] + dg.v[e]) -
                                        nn_act<Real>(m, p0[u].v[e], g0[u].v[e]);
-                        *(V4 *)(dpre + 4 * (int64_t)k) = dp;
-                        if (m.has_gate) *(V4 *)(dgpre + 4 * (int64_t)k) = dg;
+                        nn_st4<Real>(dpre + 4 * (int64_t)k, dp);
+                        if (m.has_gate) nn_st4<Real>(dgpre + 4 * (int64_t)k, dg);
                         if (last) {
 #pragma unroll
                             for (int e = 0; e < 4; ++e) part += (double)(dhv.v[e] * hw[d + e]);
                         } else {
-                            *(V4 *)(dh + 4 * (int64_t)k) = dhv;
+                            nn_st4<Real>(dh + 4 * (int64_t)k, dhv);
                         }
                     }
                 }
@@ -731,24 +765,24 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 }
                 for (int l = 0; l < L; ++l) {
                     const int D = m.dd[l];
-                    const Real *dpre = scr + m.soff_dpre[l];
-                    const Real *dgpre = scr + m.soff_dgpre[l];
+                    const NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
+                    const NCE_GLOBAL Real *dgpre = scr + m.soff_dgpre[l];
                     if (sizeof(Real) == 4 && (D & 3) == 0) {
                         // four channels per thread: 16-byte rows of the scratch and the cache
                         using V4 = NNv4<Real>;
                         const int D4 = D >> 2;
                         const int d4sh = (D4 & (D4 - 1)) == 0 ? __builtin_ctz((unsigned)D4) : -1;
                         for (int g = 0; g < (m.has_gate ? 2 : 1); ++g) {
-                            Real *cbase = cache + (g == 0 ? m.off_pre[l] : m.off_gpre[l]);
-                            const Real *dbase = g == 0 ? dpre : dgpre;
+                            NCE_GLOBAL Real *cbase = cache + (g == 0 ? m.off_pre[l] : m.off_gpre[l]);
+                            const NCE_GLOBAL Real *dbase = g == 0 ? dpre : dgpre;
                             for (int k = tid; k < counts[l + 1] * D4; k += NN_BS) {
                                 const int s = d4sh >= 0 ? k >> d4sh : k / D4, d = (k - s * D4) * 4;
-                                V4 *cp = (V4 *)(cbase + (int64_t)slot_site[s] * D + d);
-                                const V4 dv = *(const V4 *)(dbase + 4 * (int64_t)k);
-                                V4 c = *cp;
+                                NCE_GLOBAL Real *cp = cbase + (int64_t)slot_site[s] * D + d;
+                                const V4 dv = nn_ld4<Real>(dbase + 4 * (int64_t)k);
+                                V4 c = nn_ld4<Real>(cp);
 #pragma unroll
                                 for (int e = 0; e < 4; ++e) c.v[e] += dv.v[e];
-                                *cp = c;
+                                nn_st4<Real>(cp, c);
                             }
                         }
                     } else {
