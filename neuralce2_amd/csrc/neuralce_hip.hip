@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <unordered_map>
 #include <unordered_set>
 #include <utility>
 #include <cstdint>
@@ -127,6 +128,17 @@ struct NNModelDev {
     double head_b, de_scale;
     int32_t ball_cap[NN_MAXL + 1];
     int32_t hash_cap;            // power of two >= 1.5 * ball_cap[L]
+    // Two flips whose last balls share no site (ib - ia outside ball_L - ball_L): the union's slots, hits and
+    // target ranges are then the same for every such pair, so they are tables built once for centre 0.
+    // pre_ok = 0: every step builds its balls in the hash table.
+    int32_t pre_ok;
+    int32_t pre_n[NN_MAXL + 1];  // one flip's ball sizes
+    const uint32_t *pre_near;    // [ceil(N / 32)] bit v set: cell vector v lies in ball_L - ball_L
+    const int4 *pre_slot;        // [2 pre_n[L]] (flip, offset) of every union slot
+    const int32_t *pre_win;      // [2 W0] union slot of every window slot
+    const int32_t *pre_list;     // per layer: hits (source slot << 8 | shell), sorted by target
+    const int32_t *pre_end;      // per layer: end of each target's range in pre_list
+    int64_t pre_list_off[NN_MAXL], pre_end_off[NN_MAXL];
     // per precision: which per-step arrays live in LDS (else in the scratch): 0 none, 1 the layer-0 change of
     // ball 0, 2 and the sorted partner hits, 3 and the projections
     int32_t lds_mode32, lds_mode64;

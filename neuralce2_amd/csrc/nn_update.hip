@@ -72,6 +72,25 @@ __device__ inline int64_t nn_site(const NNModelDev &m, int64_t center, int4 e) {
     return ((int64_t)k0 * m.d1 + k1) * m.d2 + k2;
 }
 
+// the cell vector ib - ia outside ball_L - ball_L: the two flips' balls share no site
+__device__ inline bool nn_balls_apart(const NNModelDev &m, int64_t ia, int64_t ib) {
+    const uint32_t d12 = (uint32_t)(m.d1 * m.d2);
+    int k[2][3];
+    for (int f = 0; f < 2; ++f) {
+        const uint32_t c = (uint32_t)(f == 0 ? ia : ib);
+        const int k0 = (int)udiv_magic(c, m.div12_m, m.div12_s, d12);
+        const int rem = (int)c - k0 * (int)d12;
+        const int k1 = (int)udiv_magic((uint32_t)rem, m.div2_m, m.div2_s, (uint32_t)m.d2);
+        k[f][0] = k0; k[f][1] = k1; k[f][2] = rem - k1 * m.d2;
+    }
+    int v0 = k[1][0] - k[0][0], v1 = k[1][1] - k[0][1], v2 = k[1][2] - k[0][2];
+    v0 += v0 < 0 ? m.d0 : 0;
+    v1 += v1 < 0 ? m.d1 : 0;
+    v2 += v2 < 0 ? m.d2 : 0;
+    const uint32_t v = (uint32_t)((v0 * m.d1 + v1) * m.d2 + v2);
+    return ((gptr(m.pre_near)[v >> 5] >> (v & 31)) & 1u) == 0;
+}
+
 // ── full cache ─────────────────────────────────────────────────────────────────────────────────────────────
 constexpr int NN_H0_SITES = 64;
 
@@ -379,6 +398,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     const int64_t inj_base = (int64_t)r * n_steps;
     int counts[NN_MAXL + 1];
     bool w_staged = false;                             // the projection matrix in LDS is the current layer's
+    bool pre_in_list = false;                          // list and cnt hold the fixed table (one-layer models)
 
     for (int64_t t = 0; t < n_steps; ++t) {
         int64_t ia, ib;
@@ -397,24 +417,39 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 ++nacc;
             }
         } else {
-            // ball 0: the two layer-0 windows
+            // balls of two flips that share no site: every slot, hit and target range comes from the fixed tables
+            const bool fixed = m.pre_ok && nn_balls_apart(m, ia, ib);
+            // ball 0: the two layer-0 windows (w_pos: the hash position of each window slot, then its slot)
             for (int k = tid; k < 2 * m.W0; k += NN_BS) {
                 const NCE_GLOBAL int *wk = win + 4 * (k < m.W0 ? k : k - m.W0);
                 const int site = (int)nn_site(m, k < m.W0 ? ia : ib, make_int4(0, wk[1], wk[2], wk[3]));
-                bool claimed;
-                const int pos = nn_hash_claim(keys, mask, site, &claimed);
-                if (claimed) {
-                    const int s = atomicAdd(&misc[0], 1);
-                    vals[pos] = s;
-                    slot_site[s] = site;
+                if (fixed) {
+                    w_pos[k] = gptr(m.pre_win)[k];
+                } else {
+                    bool claimed;
+                    const int pos = nn_hash_claim(keys, mask, site, &claimed);
+                    if (claimed) {
+                        const int s = atomicAdd(&misc[0], 1);
+                        vals[pos] = s;
+                        slot_site[s] = site;
+                    }
+                    w_pos[k] = pos;
                 }
                 w_site[k] = site;
-                w_pos[k] = pos;
                 w_sp[k] = occ[site];
             }
+            if (fixed) {
+                const NCE_GLOBAL int *ps = (const NCE_GLOBAL int *)gptr(m.pre_slot);
+                for (int s = tid; s < 2 * m.pre_n[L]; s += NN_BS) {
+                    const NCE_GLOBAL int *e = ps + 4 * s;
+                    slot_site[s] = (int)nn_site(m, e[0] ? ib : ia, make_int4(0, e[1], e[2], e[3]));
+                }
+            }
             __syncthreads();
-            counts[0] = misc[0];
+            counts[0] = fixed ? 2 * m.pre_n[0] : misc[0];
             for (int k = tid; k < counts[0] * C0; k += NN_BS) dh0[k] = (Real)0;
+            if (!fixed)
+                for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = vals[w_pos[k]];
             __syncthreads();
             // layer-0 change of flip 1, then of flip 2 with flip 1 applied
             for (int f = 0; f < 2; ++f) {
@@ -443,7 +478,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         const Real v = (Real)(dv / sd[col]);
                         for (int q = 0; q < 4; ++q) {
                             if (xs[q] < 0) continue;
-                            const int s = vals[w_pos[xs[q]]];
+                            const int s = w_pos[xs[q]];
                             atomicAdd((Real *)&dh0[(int64_t)s * C0 + col], v);
                         }
                     }
@@ -591,8 +626,24 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             if (q0 + i < prev && j0 + u < M) pr[(int64_t)(q0 + i) * M + j0 + u] = acc[i][u];
                 }
                 }
-                // (2) partner sites of ball_l: claim (new sites get the next slots), then slot of every hit
+                // (2) partner sites of ball_l, (3) the hits sorted by target: list holds (source slot << 8 | shell)
+                // grouped by target, cnt[s] the end of target s's range
                 const int nh = prev * n_part;
+                int nb;
+                if (fixed) {
+                    nb = 2 * m.pre_n[l + 1];
+                    counts[l + 1] = nb;
+                    if (!pre_in_list) {
+                        const NCE_GLOBAL int *hl = gptr(m.pre_list) + m.pre_list_off[l];
+                        const NCE_GLOBAL int *he = gptr(m.pre_end) + m.pre_end_off[l];
+                        for (int i = tid; i < nh; i += NN_BS) list[i] = hl[i];
+                        for (int s = tid; s < nb; s += NN_BS) cnt[s] = he[s];
+                    }
+                    pre_in_list = L == 1;                // one layer: the table stays in place until a hash step
+                    __syncthreads();
+                } else {
+                pre_in_list = false;
+                // claim (new sites get the next slots), then the slot of every hit
                 // hit i = (q, kk) = (i / n_part, i % n_part), stepped without divisions
                 const int q_0 = n_part > 0 ? tid / n_part : 0, kk_0 = tid - q_0 * n_part;
                 const int dq = n_part > 0 ? NN_BS / n_part : 0, dk = NN_BS - dq * n_part;
@@ -610,7 +661,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     tgt[i] = pos;
                 }
                 __syncthreads();
-                const int nb = misc[0];
+                nb = misc[0];
                 counts[l + 1] = nb;
                 for (int s = tid; s < nb; s += NN_BS) cnt[s] = 0;
                 __syncthreads();
@@ -620,7 +671,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     atomicAdd(&cnt[s], 1);
                 }
                 __syncthreads();
-                // (3) counting sort of the hits by target: after the fill cnt[s] is the end of target s's range
+                // counting sort of the hits by target: after the fill cnt[s] is the end of target s's range
                 nn_block_scan(cnt, nb, red);
                 q = q_0; kk = kk_0;
                 for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
@@ -629,6 +680,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     list[pos] = (q << 8) | s_part_shell[kk];
                 }
                 __syncthreads();
+                }
                 // (4) change of the pre-activations, new activations, change of the layer output
                 NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
                 NCE_GLOBAL Real *dgpre = scr + m.soff_dgpre[l];
@@ -802,9 +854,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 E += dE;
                 ++nacc;
             }
-            // clear the table for the next step
-            for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
-            if (tid == 0) misc[0] = 0;
+            // clear the table for the next step (a step on the fixed tables leaves it empty)
+            if (!fixed) {
+                for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
+                if (tid == 0) misc[0] = 0;
+            }
         }
         if (job.traj && eval_dE == nullptr && ((t + 1) % job.sample_interval) == 0 && tid == 0)
             job.traj[((t + 1) / job.sample_interval - 1) * job.n_replicas + r] = E;
@@ -949,6 +1003,7 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
         };
         std::unordered_set<int64_t> ball;
         std::vector<int64_t> cur;
+        int64_t n1[NN_MAXL + 1];
         for (int k = 0; k < m.W0; ++k)
             if (ball.insert(site_of(0, d->window + 4 * k)).second) cur.push_back(site_of(0, d->window + 4 * k));
         for (int l = 0; l <= m.L; ++l) {
@@ -960,9 +1015,83 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
                         if (ball.insert(y).second) cur.push_back(y);
                     }
             }
+            n1[l] = (int64_t)cur.size();
             m.ball_cap[l] = (int32_t)std::min<int64_t>(2 * (int64_t)cur.size(), N);
             if ((int64_t)m.ball_cap[l] >= ((int64_t)1 << 23))
                 return set_err(NCE_ERR_UNSUPPORTED, "receptive field too large");
+        }
+        // Fixed-ball tables.  cur lists one flip's ball_L around site 0, each ball_l a prefix of n1[l] sites; by
+        // translation the ball of any centre c is c + cur.  The union of two disjoint balls numbers its slots by
+        // layer, flip 1 before flip 2 within a layer, so ball_l is the prefix 2 n1[l] of the union as the kernel
+        // requires.  The tables are skipped where two balls cannot be apart on this cell or are too large to pair.
+        const char *fixed_env = std::getenv("NCE_NN_FIXED_BALLS");
+        const int64_t nL = n1[m.L];
+        if (m.L > 0 && n_part > 0 && 2 * nL <= N && nL <= 8192 && !(fixed_env && fixed_env[0] == '0')) {
+            std::unordered_map<int64_t, int32_t> idx;
+            for (int64_t j = 0; j < nL; ++j) idx.emplace(cur[j], (int32_t)j);
+            auto uslot = [&](int f, int64_t j) -> int32_t {
+                int l = 0;
+                while (j >= n1[l]) ++l;
+                const int64_t lo = l > 0 ? n1[l - 1] : 0;
+                return (int32_t)(2 * lo + f * (n1[l] - lo) + (j - lo));
+            };
+            auto coords = [&](int64_t s, int64_t &k0, int64_t &k1, int64_t &k2) {
+                k2 = s % m.d2; k1 = (s / m.d2) % m.d1; k0 = s / ((int64_t)m.d1 * m.d2);
+            };
+            std::vector<int4> slot(2 * nL);
+            std::vector<int32_t> src_f(2 * nL), src_j(2 * nL);
+            for (int f = 0; f < 2; ++f)
+                for (int64_t j = 0; j < nL; ++j) {
+                    int64_t k0, k1, k2;
+                    coords(cur[j], k0, k1, k2);
+                    const int32_t u = uslot(f, j);
+                    slot[u] = make_int4(f, (int)k0, (int)k1, (int)k2);
+                    src_f[u] = f;
+                    src_j[u] = (int32_t)j;
+                }
+            std::vector<int32_t> win_slot(2 * (size_t)m.W0);
+            for (int f = 0; f < 2; ++f)
+                for (int k = 0; k < m.W0; ++k) win_slot[(size_t)f * m.W0 + k] = uslot(f, idx.at(site_of(0, d->window + 4 * k)));
+            std::vector<int32_t> list, ends;
+            for (int l = 0; l < m.L; ++l) {
+                // hits in the kernel's order (source q, partner k), counted by target, then placed stably
+                const int64_t nsrc = 2 * n1[l], ntgt = 2 * n1[l + 1];
+                std::vector<int32_t> tgt((size_t)(nsrc * n_part)), start((size_t)ntgt + 1, 0);
+                for (int64_t q = 0; q < nsrc; ++q)
+                    for (int k = 0; k < n_part; ++k) {
+                        const int64_t y = site_of(cur[src_j[q]], d->partners + 4 * k);
+                        const int32_t t = uslot(src_f[q], idx.at(y));
+                        tgt[(size_t)(q * n_part + k)] = t;
+                        ++start[(size_t)t + 1];
+                    }
+                for (int64_t t = 0; t < ntgt; ++t) start[(size_t)t + 1] += start[(size_t)t];
+                m.pre_end_off[l] = (int64_t)ends.size();
+                ends.insert(ends.end(), start.begin() + 1, start.end());
+                m.pre_list_off[l] = (int64_t)list.size();
+                list.resize(list.size() + (size_t)(nsrc * n_part));
+                int32_t *lp = list.data() + m.pre_list_off[l];
+                for (int64_t q = 0; q < nsrc; ++q)
+                    for (int k = 0; k < n_part; ++k)
+                        lp[start[(size_t)tgt[(size_t)(q * n_part + k)]]++] = (int32_t)(q << 8) | part_shell[k];
+            }
+            // ball_L - ball_L on the cell
+            std::vector<uint32_t> near((size_t)((N + 31) / 32), 0u);
+            std::vector<int64_t> c0(nL), c1(nL), c2(nL);
+            for (int64_t j = 0; j < nL; ++j) coords(cur[j], c0[j], c1[j], c2[j]);
+            for (int64_t a = 0; a < nL; ++a)
+                for (int64_t b = 0; b < nL; ++b) {
+                    const int64_t v0 = (c0[a] - c0[b] + m.d0) % m.d0, v1 = (c1[a] - c1[b] + m.d1) % m.d1,
+                                  v2 = (c2[a] - c2[b] + m.d2) % m.d2;
+                    const int64_t v = (v0 * m.d1 + v1) * m.d2 + v2;
+                    near[(size_t)(v >> 5)] |= 1u << (v & 31);
+                }
+            for (int l = 0; l <= m.L; ++l) m.pre_n[l] = (int32_t)n1[l];
+            UP(near.data(), near.size(), &m.pre_near);
+            UP(slot.data(), slot.size(), &m.pre_slot);
+            UP(win_slot.data(), win_slot.size(), &m.pre_win);
+            UP(list.data(), list.size(), &m.pre_list);
+            UP(ends.data(), ends.size(), &m.pre_end);
+            m.pre_ok = 1;
         }
     }
     int hc = 64;

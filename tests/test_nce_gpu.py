@@ -242,3 +242,72 @@ def test_setup_local_delta_and_commit(torch_dev, ptni_nce):
     carried = cache.cache.clone()
     cache.anchor()
     assert (carried - cache.cache).abs().max().item() < 1e-9
+
+
+def _apart(nn, ia, ib):
+    """True where the two flips' last-layer balls share no site on the cell: the cell vector ib - ia lies outside
+    ball_L - ball_L (one flip's ball: the window dilated depth times by the partner offsets, sites deduplicated)."""
+    d = np.array([int(x) for x in nn.geo["d"]], np.int64)
+    ball = np.unique(nn.geo["window"][:, 1:].astype(np.int64) % d, axis=0)
+    part = nn.geo["partners"][:, 1:].astype(np.int64)
+    for _ in range(nn.depth):
+        ball = np.unique(np.concatenate([ball, ((ball[:, None, :] + part[None]) % d).reshape(-1, 3)]), axis=0)
+    diff = ((ball[:, None, :] - ball[None, :, :]) % d).reshape(-1, 3)
+    near = np.zeros(nn.N, bool)
+    near[(diff[:, 0] * d[1] + diff[:, 1]) * d[2] + diff[:, 2]] = True
+    co = lambda x: np.stack([x // (d[1] * d[2]), (x // d[2]) % d[1], x % d[2]], -1)
+    v = (co(ib) - co(ia)) % d
+    return ~near[(v[..., 0] * d[1] + v[..., 1]) * d[2] + v[..., 2]]
+
+
+@pytest.mark.parametrize("kind,n,prec,steps", [("ptni", 12, 0, 150), ("ptni", 12, 1, 150), ("silu", 20, 0, 40)])
+def test_fixed_balls_match_hash_path(torch_dev, ptni_nce, monkeypatch, kind, n, prec, steps):
+    """Swaps whose balls share no site run on tables built once per model (slots, hits sorted by target); the rest
+    build their balls in the LDS hash table.  The same injected stream with the tables switched off
+    (NCE_NN_FIXED_BALLS=0, every step on the hash path, which the oracle tests above pin on small cells) gives the
+    same occupations and acceptances and the same energies to the precision's bar, and the running totals and caches
+    of the table run equal a full recompute (sizes where the oracle's brute-force placement is too slow)."""
+    import torch
+    from neuralce2_amd.lattice import bulk_fcc
+    from neuralce2_amd.nce.mc_model import NNModel, silu_network
+    from neuralce2_amd.nce.mcmc import NNJobState
+    at = bulk_fcc("Ni", 3.924, cubic=True).repeat((n, n, n))
+    rng = np.random.default_rng(5)
+    Z = np.full(len(at), 28); Z[rng.choice(len(at), len(at) // 2, replace=False)] = 78; at.numbers = Z
+    if kind == "ptni":
+        mdl, cs, P, mu, sd = ptni_nce
+        make = lambda: NNModel(cs, at, P, mu, sd, mdl["shell_edges"], ys=mdl["ys"], ym=mdl["ym"])
+        tol_e = 1e-10 if prec == 0 else 2e-5 / mdl["ys"]
+    else:
+        from neuralce2_amd.synthetic import fcc_cluster_space
+        cs = fcc_cluster_space((9.0, 6.0, 5.0))
+        P = silu_network(len(cs), 64, 3, 1, seed=0)
+        make = lambda: NNModel(cs, at, P, np.zeros(len(cs)), np.ones(len(cs)), [0.0, 3.3], act="silu", readout="sum")
+        tol_e = 1e-8
+    monkeypatch.setenv("NCE_NN_FIXED_BALLS", "0")
+    nn_hash = make()
+    monkeypatch.delenv("NCE_NN_FIXED_BALLS")
+    nn_fix = make()
+    perm, N = nn_fix.geo["perm"], nn_fix.N
+    R = 4 if kind == "ptni" else 2
+    sig_o = (at.numbers == 78).astype(np.int8)
+    occ_o = np.stack([rng.permutation(sig_o) for _ in range(R)])
+    betas = (1.0 / (8.617333262e-5 * np.geomspace(500, 1500, R))).astype(np.float32).astype(np.float64)
+    ia = rng.integers(0, N, (R, steps)); ib = rng.integers(0, N, (R, steps)); u = rng.random((R, steps))
+    apart = _apart(nn_fix, ia, ib) & (ia != ib)
+    assert 10 < apart.sum() < apart.size - 10, apart.sum()       # both kinds of step occur
+    inj = tuple(torch.tensor(x, device=torch_dev) for x in (ia, ib, u))
+    runs = []
+    for nn in (nn_hash, nn_fix):
+        st = NNJobState(nn, occ_o[:, perm], betas, precision=prec, sample_interval=1, samples_per_chunk=steps)
+        st.run(steps, inj)
+        torch.cuda.synchronize()
+        runs.append(st)
+    a, b = runs
+    assert np.array_equal(a.occ.cpu().numpy(), b.occ.cpu().numpy())
+    assert np.array_equal(a.n_accept.cpu().numpy(), b.n_accept.cpu().numpy())
+    assert np.max(np.abs(a.energy.cpu().numpy() - b.energy.cpu().numpy())) < tol_e
+    cache_chain = b.cache.clone()
+    E_run, E_full = b.anchor()
+    assert torch.max(torch.abs(E_run - E_full)).item() < tol_e
+    assert torch.max(torch.abs(cache_chain - b.cache)).item() < (1e-9 if prec == 0 else 1e-4)
