@@ -118,6 +118,7 @@ struct NNModelDev {
     const uint2 *ent_slot;       // four u16 slots, padding = 0xFFFF
     const int32_t *ent_col;
     const double *ent_coef;
+    const double *ent_q;         // [J0] coefficient / sd of the entry's channel
     const int32_t *shell_count;  // [P]
     const int32_t *part_shell;   // [n_part] shell of each partner
     const int4 *part;            // [n_part]

@@ -390,7 +390,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     const NCE_GLOBAL int *win = (const NCE_GLOBAL int *)gptr(m.win);
     const NCE_GLOBAL uint64_t *ent_slot = (const NCE_GLOBAL uint64_t *)gptr(m.ent_slot);
     const NCE_GLOBAL int *ent_col = gptr(m.ent_col);
-    const NCE_GLOBAL double *ent_coef = gptr(m.ent_coef), *sd = gptr(m.sd);
+    const NCE_GLOBAL double *ent_q = gptr(m.ent_q);
     double E = job.energy[r];
     const double beta = job.beta[r];
     const uint64_t step0 = (uint64_t)job.step[r];
@@ -399,6 +399,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int counts[NN_MAXL + 1];
     bool w_staged = false;                             // the projection matrix in LDS is the current layer's
     bool pre_in_list = false;                          // list and cnt hold the fixed table (one-layer models)
+    // this thread's first layer-0 entry (e = tid) is the same in every step: held in registers for the launch
+    const bool own_e = tid < m.J0;
+    const uint64_t own_sl = own_e ? ent_slot[tid] : 0;
+    const double own_q = own_e ? ent_q[tid] : 0.0;
+    const int own_col = own_e ? ent_col[tid] : 0;
 
     for (int64_t t = 0; t < n_steps; ++t) {
         int64_t ia, ib;
@@ -456,7 +461,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const int64_t fs = f == 0 ? ia : ib;
                 const int wb = f == 0 ? 0 : m.W0;          // this flip's window in w_site / w_pos / w_sp
                 for (int e = tid; e < m.J0; e += NN_BS) {
-                    const uint64_t sl = ent_slot[e];
+                    const bool own = e == tid;
+                    const uint64_t sl = own ? own_sl : ent_slot[e];
                     const uint32_t lo = (uint32_t)sl, hi = (uint32_t)(sl >> 32);
                     const uint32_t s4[4] = {lo & 0xffffu, lo >> 16, hi & 0xffffu, hi >> 16};
                     int xs[4];
@@ -472,10 +478,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         po *= sp ? 1.0 : -1.0;
                         pn *= spn ? 1.0 : -1.0;
                     }
-                    const double dv = (pn - po) * ent_coef[e];
+                    // (pn - po) is 0 or +-2, so (pn - po) * (coef / sd) rounds as ((pn - po) * coef) / sd does
+                    const double dv = (pn - po) * (own ? own_q : ent_q[e]);
                     if (dv != 0.0) {
-                        const int col = ent_col[e];
-                        const Real v = (Real)(dv / sd[col]);
+                        const int col = own ? own_col : ent_col[e];
+                        const Real v = (Real)dv;
                         for (int q = 0; q < 4; ++q) {
                             if (xs[q] < 0) continue;
                             const int s = w_pos[xs[q]];
@@ -486,7 +493,16 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             }
             __syncthreads();
             double part = 0.0;
-            for (int k = tid; k < counts[0] * C0; k += NN_BS) part += (double)(dh0[k] * lin[k % C0]);
+            {
+                // channel of element k = k % C0, stepped without divisions
+                const int dcs = NN_BS % C0;
+                int c = tid % C0;
+                for (int k = tid; k < counts[0] * C0; k += NN_BS) {
+                    part += (double)(dh0[k] * lin[c]);
+                    c += dcs;
+                    c -= c >= C0 ? C0 : 0;
+                }
+            }
             // hidden layers: projection of the previous layer's change, partner hits sorted by target, gather
             for (int l = 0; l < L; ++l) {
                 const int Cin = l == 0 ? m.c0_eff : m.cin[l], D = m.dd[l], M = m.mcat[l];
@@ -1132,6 +1148,9 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     UP(slots.data(), slots.size(), &m.ent_slot);
     UP(d->entry_col, (size_t)m.J0, &m.ent_col);
     UP(d->entry_coef, (size_t)m.J0, &m.ent_coef);
+    std::vector<double> ent_q(std::max(m.J0, 1), 0.0);
+    for (int e = 0; e < m.J0; ++e) ent_q[e] = d->entry_coef[e] / d->sd[d->entry_col[e]];
+    UP(ent_q.data(), ent_q.size(), &m.ent_q);
     std::vector<int32_t> sc(std::max(m.P, 1), 0);
     for (int p = 0; p < m.P; ++p) sc[p] = d->shell_count[p];
     UP(sc.data(), sc.size(), &m.shell_count);
