@@ -196,7 +196,10 @@ typedef struct {
     const double *inj_u;
 } nce_nn_job;
 
-/* Per-replica sizes (elements of the precision's type) of the cache and the scratch of a NeuralCE model. */
+/* Upload a model.  Besides the receptive-field sizes the library builds, for this cell, the slots and sorted
+ * partner hits of two flips whose balls share no site (the same for every such pair); NCE_NN_FIXED_BALLS=0 in the
+ * environment at this call leaves them out, and every step then builds its balls in the LDS hash table.
+ * nce_nn_sizes: per-replica sizes (elements of the precision's type) of the cache and the scratch. */
 int nce_nn_model_create(nce_ctx *ctx, const nce_nn_model_desc *desc, int32_t *model_id);
 int nce_nn_sizes(nce_ctx *ctx, int32_t model, int64_t *cache_elems, int64_t *scratch_elems);
 
