@@ -399,11 +399,6 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int counts[NN_MAXL + 1];
     bool w_staged = false;                             // the projection matrix in LDS is the current layer's
     bool pre_in_list = false;                          // list and cnt hold the fixed table (one-layer models)
-    // this thread's first layer-0 entry (e = tid) is the same in every step: held in registers for the launch
-    const bool own_e = tid < m.J0;
-    const uint64_t own_sl = own_e ? ent_slot[tid] : 0;
-    const double own_q = own_e ? ent_q[tid] : 0.0;
-    const int own_col = own_e ? ent_col[tid] : 0;
 
     for (int64_t t = 0; t < n_steps; ++t) {
         int64_t ia, ib;
@@ -461,8 +456,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const int64_t fs = f == 0 ? ia : ib;
                 const int wb = f == 0 ? 0 : m.W0;          // this flip's window in w_site / w_pos / w_sp
                 for (int e = tid; e < m.J0; e += NN_BS) {
-                    const bool own = e == tid;
-                    const uint64_t sl = own ? own_sl : ent_slot[e];
+                    const uint64_t sl = ent_slot[e];
                     const uint32_t lo = (uint32_t)sl, hi = (uint32_t)(sl >> 32);
                     const uint32_t s4[4] = {lo & 0xffffu, lo >> 16, hi & 0xffffu, hi >> 16};
                     int xs[4];
@@ -479,9 +473,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         pn *= spn ? 1.0 : -1.0;
                     }
                     // (pn - po) is 0 or +-2, so (pn - po) * (coef / sd) rounds as ((pn - po) * coef) / sd does
-                    const double dv = (pn - po) * (own ? own_q : ent_q[e]);
+                    const double dv = (pn - po) * ent_q[e];
                     if (dv != 0.0) {
-                        const int col = own ? own_col : ent_col[e];
+                        const int col = ent_col[e];
                         const Real v = (Real)dv;
                         for (int q = 0; q < 4; ++q) {
                             if (xs[q] < 0) continue;
