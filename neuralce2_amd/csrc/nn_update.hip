@@ -221,7 +221,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
     L.slot_site = o; o = align16(o + (size_t)ball_cap * sizeof(int));
     L.cnt = o;       o = align16(o + (size_t)(ball_cap + 1) * sizeof(int));
     L.red = o;       o = align16(o + (NN_BS / 64 + 1) * sizeof(double));
-    L.misc = o;      o = align16(o + 4 * sizeof(int));
+    L.misc = o;      o = align16(o + (4 + NN_MAXL + 1) * sizeof(int));
     L.wv = o;        o = align16(o + (size_t)win_n * 3 * sizeof(int));
     L.part = o;      o = align16(o + (size_t)n_part * (sizeof(int4) + sizeof(int)) + (size_t)n_c0 * sizeof(int));
     L.w = o;         o = align16(o + w_bytes);
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     const uint64_t step0 = (uint64_t)job.step[r];
     int64_t nacc = 0;
     const int64_t inj_base = (int64_t)r * n_steps;
-    int counts[NN_MAXL + 1];
+    int *counts = misc + 4;                            // slots of ball_0 .. ball_L this step (same in every thread)
     bool w_staged = false;                             // the projection matrix in LDS is the current layer's
     bool pre_in_list = false;                          // list and cnt hold the fixed table (one-layer models)
 
