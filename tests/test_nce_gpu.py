@@ -311,3 +311,23 @@ def test_fixed_balls_match_hash_path(torch_dev, ptni_nce, monkeypatch, kind, n, 
     E_run, E_full = b.anchor()
     assert torch.max(torch.abs(E_run - E_full)).item() < tol_e
     assert torch.max(torch.abs(cache_chain - b.cache)).item() < (1e-9 if prec == 0 else 1e-4)
+    # the evaluation entry point (nce_nn_delta: the energy change of one swap per replica, nothing committed) on
+    # both paths, each replica's swap one whose balls share no site
+    occ_now = b.occ.cpu().numpy()
+    cand = rng.integers(0, N, (R, 64, 2))
+    ok = _apart(nn_fix, cand[..., 0], cand[..., 1]) & \
+        (np.take_along_axis(occ_now, cand[..., 0], 1) != np.take_along_axis(occ_now, cand[..., 1], 1))
+    assert ok.any(axis=1).all()
+    pick = cand[np.arange(R), ok.argmax(axis=1)]
+    inj1 = (torch.tensor(pick[:, :1], device=torch_dev), torch.tensor(pick[:, 1:], device=torch_dev),
+            torch.zeros((R, 1), dtype=torch.float64, device=torch_dev))
+    dEs = []
+    for st in runs:
+        occ_before = st.occ.clone()
+        dE = torch.empty(R, dtype=torch.float64, device=torch_dev)
+        st.nn.ctx.nn_delta([st.job(inj1)], dE, prec)
+        torch.cuda.synchronize()
+        assert torch.equal(st.occ, occ_before)
+        dEs.append(dE.cpu().numpy())
+    assert np.all(dEs[1] != 0.0)
+    assert np.max(np.abs(dEs[0] - dEs[1])) < tol_e
