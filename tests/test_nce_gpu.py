@@ -260,7 +260,8 @@ def _apart(nn, ia, ib):
     return ~near[(v[..., 0] * d[1] + v[..., 1]) * d[2] + v[..., 2]]
 
 
-@pytest.mark.parametrize("kind,n,prec,steps", [("ptni", 12, 0, 150), ("ptni", 12, 1, 150), ("silu", 20, 0, 40)])
+@pytest.mark.parametrize("kind,n,prec,steps", [("ptni", 12, 0, 150), ("ptni", 12, 1, 150), ("ptni", 12, 2, 150),
+                                               ("silu", 20, 0, 40), ("silu", 20, 1, 40)])
 def test_fixed_balls_match_hash_path(torch_dev, ptni_nce, monkeypatch, kind, n, prec, steps):
     """Swaps whose balls share no site run on tables built once per model (slots, hits sorted by target); the rest
     build their balls in the LDS hash table.  The same injected stream with the tables switched off
@@ -283,7 +284,7 @@ def test_fixed_balls_match_hash_path(torch_dev, ptni_nce, monkeypatch, kind, n, 
         cs = fcc_cluster_space((9.0, 6.0, 5.0))
         P = silu_network(len(cs), 64, 3, 1, seed=0)
         make = lambda: NNModel(cs, at, P, np.zeros(len(cs)), np.ones(len(cs)), [0.0, 3.3], act="silu", readout="sum")
-        tol_e = 1e-8
+        tol_e = 1e-8 if prec == 0 else 1e-3
     monkeypatch.setenv("NCE_NN_FIXED_BALLS", "0")
     nn_hash = make()
     monkeypatch.delenv("NCE_NN_FIXED_BALLS")
@@ -307,10 +308,11 @@ def test_fixed_balls_match_hash_path(torch_dev, ptni_nce, monkeypatch, kind, n, 
     assert np.array_equal(a.occ.cpu().numpy(), b.occ.cpu().numpy())
     assert np.array_equal(a.n_accept.cpu().numpy(), b.n_accept.cpu().numpy())
     assert np.max(np.abs(a.energy.cpu().numpy() - b.energy.cpu().numpy())) < tol_e
-    cache_chain = b.cache.clone()
-    E_run, E_full = b.anchor()
-    assert torch.max(torch.abs(E_run - E_full)).item() < tol_e
-    assert torch.max(torch.abs(cache_chain - b.cache)).item() < (1e-9 if prec == 0 else 1e-4)
+    if prec < 2:                 # BF16 projections drift from a full FP32 rebuild by design (re-anchored by drivers)
+        cache_chain = b.cache.clone()
+        E_run, E_full = b.anchor()
+        assert torch.max(torch.abs(E_run - E_full)).item() < tol_e
+        assert torch.max(torch.abs(cache_chain - b.cache)).item() < (1e-9 if prec == 0 else 1e-4)
     # the evaluation entry point (nce_nn_delta: the energy change of one swap per replica, nothing committed) on
     # both paths, each replica's swap one whose balls share no site
     occ_now = b.occ.cpu().numpy()
