@@ -700,7 +700,82 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const bool last = l + 1 == L;
                 const int og = D * (1 + P);
                 constexpr int U = 4;                     // items per thread in flight together
-                if (sizeof(Real) == 4 && (D & 3) == 0) {
+                if (sizeof(Real) == 4 && MODE < 3 && (D & 7) == 0) {
+                // projections in the scratch: eight channels per item (FP32), two 16-byte halves, so each hit's list
+                // entry is read from memory once for both (with the projections in LDS four channels per item and
+                // more hits in flight do better)
+                using V4 = NNv4<Real>;
+                const V4 z4 = V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
+                const int D8 = D >> 3;
+                const int d8sh = (D8 & (D8 - 1)) == 0 ? __builtin_ctz((unsigned)D8) : -1;
+                for (int k = tid; k < nb * D8; k += NN_BS) {
+                    const int s = d8sh >= 0 ? k >> d8sh : k / D8, d = (k - s * D8) * 8;
+                    const int64_t o = (int64_t)slot_site[s] * D + d;
+                    const V4 pa = nn_ld4<Real>(cpre + o), pb = nn_ld4<Real>(cpre + o + 4);
+                    V4 ga = z4, gb = z4;
+                    if (m.has_gate) { ga = nn_ld4<Real>(cgpre + o); gb = nn_ld4<Real>(cgpre + o + 4); }
+                    V4 dpa = z4, dpb = z4, dga = z4, dgb = z4;
+                    if (s < prev) { dpa = nn_ld4<Real>(pr + (int64_t)s * M + d); dpb = nn_ld4<Real>(pr + (int64_t)s * M + d + 4); }
+                    const int j0 = s == 0 ? 0 : cnt[s - 1], j1 = cnt[s];
+                    auto at = [&](int h) -> int64_t { return (int64_t)(h >> 8) * M + D * (h & 0xff) + d; };
+                    auto rd = [&](int64_t a) -> V4 { return nn_ld4<Real>(pr + a); };
+                    int j = j0;
+                    for (; j + 2 <= j1; j += 2) {
+                        const int64_t a0 = at(list[j]), a1 = at(list[j + 1]);
+                        const V4 x0 = rd(a0 + D), x1 = rd(a1 + D), x2 = rd(a0 + D + 4), x3 = rd(a1 + D + 4);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            dpa.v[e] += x0.v[e] + x1.v[e];
+                            dpb.v[e] += x2.v[e] + x3.v[e];
+                        }
+                        if (m.has_gate) {
+                            const V4 y0 = rd(a0 + og), y1 = rd(a1 + og), y2 = rd(a0 + og + 4), y3 = rd(a1 + og + 4);
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                dga.v[e] += y0.v[e] + y1.v[e];
+                                dgb.v[e] += y2.v[e] + y3.v[e];
+                            }
+                        }
+                    }
+                    if (j < j1) {
+                        const int64_t a = at(list[j]);
+                        const V4 x0 = rd(a + D), x2 = rd(a + D + 4);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            dpa.v[e] += x0.v[e];
+                            dpb.v[e] += x2.v[e];
+                        }
+                        if (m.has_gate) {
+                            const V4 y0 = rd(a + og), y2 = rd(a + og + 4);
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                dga.v[e] += y0.v[e];
+                                dgb.v[e] += y2.v[e];
+                            }
+                        }
+                    }
+                    V4 ha, hb;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        ha.v[e] = nn_act<Real>(m, pa.v[e] + dpa.v[e], ga.v[e] + dga.v[e]) - nn_act<Real>(m, pa.v[e], ga.v[e]);
+                        hb.v[e] = nn_act<Real>(m, pb.v[e] + dpb.v[e], gb.v[e] + dgb.v[e]) - nn_act<Real>(m, pb.v[e], gb.v[e]);
+                    }
+                    // scratch rows by target: item k covers elements 8k .. 8k + 7
+                    nn_st4<Real>(dpre + 8 * (int64_t)k, dpa);
+                    nn_st4<Real>(dpre + 8 * (int64_t)k + 4, dpb);
+                    if (m.has_gate) {
+                        nn_st4<Real>(dgpre + 8 * (int64_t)k, dga);
+                        nn_st4<Real>(dgpre + 8 * (int64_t)k + 4, dgb);
+                    }
+                    if (last) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) part += (double)(ha.v[e] * hw[d + e]) + (double)(hb.v[e] * hw[d + 4 + e]);
+                    } else {
+                        nn_st4<Real>(dh + 8 * (int64_t)k, ha);
+                        nn_st4<Real>(dh + 8 * (int64_t)k + 4, hb);
+                    }
+                }
+                } else if (sizeof(Real) == 4 && (D & 3) == 0) {
                 // four channels per item (FP32): every hit's projection block is read 16 bytes at a time
                 using V4 = NNv4<Real>;
                 constexpr int U2 = 2;                    // items (four channels each) per thread in flight together
