@@ -524,6 +524,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const int lane = tid & 63, wv = tid >> 6;
                     const int nrt = (prev + 15) >> 4, nct = M >> 4;
                     constexpr int CT = 4;                // column tiles per pass (A fragment loaded once per pass)
+                    // deeper layers read their input rows from the scratch: eight consecutive channels per lane
+                    const bool rows_v = sizeof(Real) == 4 && l > 0 && (Cin & 7) == 0;
                     for (int task = wv; task < nrt * ((nct + CT - 1) / CT); task += NN_BS / 64) {
                         const int rt = task % nrt, cg = (task / nrt) * CT;
                         const int row = rt * 16 + (lane & 15), kb = 8 * (lane >> 4);
@@ -532,12 +534,26 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         for (int t = 0; t < CT; ++t) acc[t] = nn_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
                         for (int k0 = 0; k0 < Kp; k0 += 32) {
                             nn_bf16x8 av;
+                            if (rows_v) {
+                                NNv4<Real> lo = NNv4<Real>{{(Real)0, (Real)0, (Real)0, (Real)0}}, hi = lo;
+                                if (row < prev && k0 + kb < Cin) {
+                                    const NCE_GLOBAL Real *rp = dxg + (int64_t)row * Cin + k0 + kb;
+                                    lo = nn_ld4<Real>(rp);
+                                    hi = nn_ld4<Real>(rp + 4);
+                                }
 #pragma unroll
-                            for (int i = 0; i < 8; ++i) {
-                                const int c = k0 + kb + i;
-                                float x = 0.0f;
-                                if (row < prev && c < Cin) x = (float)dx(row, c);
-                                av[i] = (__bf16)x;
+                                for (int i = 0; i < 4; ++i) {
+                                    av[i] = (__bf16)(float)lo.v[i];
+                                    av[4 + i] = (__bf16)(float)hi.v[i];
+                                }
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < 8; ++i) {
+                                    const int c = k0 + kb + i;
+                                    float x = 0.0f;
+                                    if (row < prev && c < Cin) x = (float)dx(row, c);
+                                    av[i] = (__bf16)x;
+                                }
                             }
 #pragma unroll
                             for (int t = 0; t < CT; ++t) {
