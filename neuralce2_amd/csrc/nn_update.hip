@@ -594,6 +594,29 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         nn_floatx4 acc[CT];
 #pragma unroll
                         for (int t = 0; t < CT; ++t) acc[t] = nn_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                        if (sizeof(Real) == 4 && l > 0 && (Cin & 15) == 0) {
+                            // input rows in the scratch: lane group kq takes the k range kq * Cin/4 .. +Cin/4 - 1, so
+                            // its next four k are consecutive channels of its row (one 16-byte load); the sum over k
+                            // is the same, only its order differs
+                            const int Kq = Cin >> 2;
+                            const NCE_GLOBAL Real *rp = dxg + (int64_t)row * Cin + kq * Kq;
+                            for (int t0 = 0; t0 < Kq; t0 += 4) {
+                                NNv4<Real> av = NNv4<Real>{{(Real)0, (Real)0, (Real)0, (Real)0}};
+                                if (row < prev) av = nn_ld4<Real>(rp + t0);
+#pragma unroll 1
+                                for (int u = 0; u < 4; ++u) {
+                                    const int c = kq * Kq + t0 + u;
+                                    const float a = (float)(u == 0 ? av.v[0] : u == 1 ? av.v[1] : u == 2 ? av.v[2] : av.v[3]);
+#pragma unroll
+                                    for (int t = 0; t < CT; ++t) {
+                                        if (cg + t < nct) {
+                                            const float b = (float)wl[c * M + (cg + t) * 16 + (lane & 15)];
+                                            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+                                        }
+                                    }
+                                }
+                            }
+                        } else {
                         for (int k0 = 0; k0 < Cin; k0 += 4) {
                             const int c = k0 + kq;
                             float a = 0.0f;
@@ -605,6 +628,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                                     acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
                                 }
                             }
+                        }
                         }
 #pragma unroll
                         for (int t = 0; t < CT; ++t) {
