@@ -658,6 +658,29 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     for (int i = 0; i < NN_RB; ++i)
 #pragma unroll
                         for (int u = 0; u < 4; ++u) acc[i][u] = (Real)0;
+                    if (l > 0 && (Cin & 1) == 0) {
+                        // input rows in the scratch: two consecutive channels of each row per load
+                        typedef Real vec2 __attribute__((ext_vector_type(2)));
+                        for (int c = 0; c < Cin; c += 2) {
+                            Real wa[4], wb[4], xa[NN_RB], xb[NN_RB];
+#pragma unroll
+                            for (int i = 0; i < NN_RB; ++i) {
+                                vec2 x = vec2{(Real)0, (Real)0};
+                                if (q0 + i < prev) x = *(const NCE_GLOBAL vec2 *)(dxg + (int64_t)(q0 + i) * Cin + c);
+                                xa[i] = x[0];
+                                xb[i] = x[1];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                wa[u] = j0 + u < M ? wl[c * M + j0 + u] : (Real)0;
+                                wb[u] = j0 + u < M ? wl[(c + 1) * M + j0 + u] : (Real)0;
+                            }
+#pragma unroll
+                            for (int i = 0; i < NN_RB; ++i)
+#pragma unroll
+                                for (int u = 0; u < 4; ++u) acc[i][u] += xa[i] * wa[u] + xb[i] * wb[u];
+                        }
+                    } else {
                     for (int c = 0; c < Cin; ++c) {
                         Real wv[4], xv[NN_RB];
 #pragma unroll
@@ -668,6 +691,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         for (int i = 0; i < NN_RB; ++i)
 #pragma unroll
                             for (int u = 0; u < 4; ++u) acc[i][u] += xv[i] * wv[u];
+                    }
                     }
 #pragma unroll
                     for (int i = 0; i < NN_RB; ++i)
