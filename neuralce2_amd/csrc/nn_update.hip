@@ -328,6 +328,27 @@ __device__ inline NNv4<Real> nn_ld4(const NCE_GLOBAL Real *p) {
     return NNv4<Real>{{x[0], x[1], x[2], x[3]}};
 }
 template <typename Real>
+struct alignas(2 * sizeof(Real)) NNv2 {
+    Real v[2];
+};
+template <typename Real>
+__device__ inline NNv2<Real> nn_ld2(const Real *p) {
+    typedef Real vec __attribute__((ext_vector_type(2)));
+    const vec x = *(const vec *)p;
+    return NNv2<Real>{{x[0], x[1]}};
+}
+template <typename Real>
+__device__ inline NNv2<Real> nn_ld2(const NCE_GLOBAL Real *p) {
+    typedef Real vec __attribute__((ext_vector_type(2)));
+    const vec x = *(const NCE_GLOBAL vec *)p;
+    return NNv2<Real>{{x[0], x[1]}};
+}
+template <typename Real>
+__device__ inline void nn_st2(NCE_GLOBAL Real *p, const NNv2<Real> &v) {
+    typedef Real vec __attribute__((ext_vector_type(2)));
+    *(NCE_GLOBAL vec *)p = vec{v.v[0], v.v[1]};
+}
+template <typename Real>
 __device__ inline void nn_st4(NCE_GLOBAL Real *p, const NNv4<Real> &v) {
     typedef Real vec __attribute__((ext_vector_type(4)));
     *(NCE_GLOBAL vec *)p = vec{v.v[0], v.v[1], v.v[2], v.v[3]};
@@ -905,6 +926,76 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             for (int e = 0; e < 4; ++e) part += (double)(dhv.v[e] * hw[d + e]);
                         } else {
                             nn_st4<Real>(dh + 4 * (int64_t)k, dhv);
+                        }
+                    }
+                }
+                } else if ((D & 1) == 0) {
+                // two channels per item (FP64, or FP32 with an even width): each hit's projection block is read
+                // two values per load
+                using V2 = NNv2<Real>;
+                constexpr int U2 = 2;                    // items (two channels each) per thread in flight together
+                const int D2 = D >> 1;
+                const int d2sh = (D2 & (D2 - 1)) == 0 ? __builtin_ctz((unsigned)D2) : -1;
+                for (int k0 = tid; k0 < nb * D2; k0 += U2 * NN_BS) {
+                    V2 p0[U2], g0[U2];
+                    int sv[U2], dv[U2];
+#pragma unroll
+                    for (int u = 0; u < U2; ++u) {
+                        const int k = k0 + u * NN_BS;
+                        const int kk = k < nb * D2 ? k : 0;
+                        sv[u] = d2sh >= 0 ? kk >> d2sh : kk / D2;
+                        dv[u] = (kk - sv[u] * D2) * 2;
+                        const int64_t o = (int64_t)slot_site[sv[u]] * D + dv[u];
+                        p0[u] = nn_ld2<Real>(cpre + o);
+                        if (m.has_gate) g0[u] = nn_ld2<Real>(cgpre + o);
+                        else g0[u] = V2{{(Real)0, (Real)0}};
+                    }
+#pragma unroll
+                    for (int u = 0; u < U2; ++u) {
+                        const int k = k0 + u * NN_BS;
+                        if (k >= nb * D2) break;
+                        const int s = sv[u], d = dv[u];
+                        V2 dp = s < prev ? nn_ld2<Real>(pr + (int64_t)s * M + d) : V2{{(Real)0, (Real)0}};
+                        V2 dg = V2{{(Real)0, (Real)0}};
+                        const int j0 = s == 0 ? 0 : cnt[s - 1], j1 = cnt[s];
+                        auto at = [&](int h) -> int64_t { return (int64_t)(h >> 8) * M + D * (h & 0xff) + d; };
+                        auto rd = [&](int64_t a) -> V2 { return nn_ld2<Real>(pr + a); };
+                        int j = j0;
+                        for (; j + 4 <= j1; j += 4) {
+                            const int64_t a0 = at(list[j]), a1 = at(list[j + 1]), a2 = at(list[j + 2]),
+                                          a3 = at(list[j + 3]);
+                            const V2 x0 = rd(a0 + D), x1 = rd(a1 + D), x2 = rd(a2 + D), x3 = rd(a3 + D);
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) dp.v[e] += (x0.v[e] + x1.v[e]) + (x2.v[e] + x3.v[e]);
+                            if (m.has_gate) {
+                                const V2 y0 = rd(a0 + og), y1 = rd(a1 + og), y2 = rd(a2 + og), y3 = rd(a3 + og);
+#pragma unroll
+                                for (int e = 0; e < 2; ++e) dg.v[e] += (y0.v[e] + y1.v[e]) + (y2.v[e] + y3.v[e]);
+                            }
+                        }
+                        for (; j < j1; ++j) {
+                            const int64_t a = at(list[j]);
+                            const V2 x = rd(a + D);
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) dp.v[e] += x.v[e];
+                            if (m.has_gate) {
+                                const V2 y = rd(a + og);
+#pragma unroll
+                                for (int e = 0; e < 2; ++e) dg.v[e] += y.v[e];
+                            }
+                        }
+                        V2 dhv;
+#pragma unroll
+                        for (int e = 0; e < 2; ++e)
+                            dhv.v[e] = nn_act<Real>(m, p0[u].v[e] + dp.v[e], g0[u].v[e] + dg.v[e]) -
+                                       nn_act<Real>(m, p0[u].v[e], g0[u].v[e]);
+                        nn_st2<Real>(dpre + 2 * (int64_t)k, dp);
+                        if (m.has_gate) nn_st2<Real>(dgpre + 2 * (int64_t)k, dg);
+                        if (last) {
+#pragma unroll
+                            for (int e = 0; e < 2; ++e) part += (double)(dhv.v[e] * hw[d + e]);
+                        } else {
+                            nn_st2<Real>(dh + 2 * (int64_t)k, dhv);
                         }
                     }
                 }
