@@ -199,6 +199,8 @@ typedef struct {
 /* Upload a model.  Besides the receptive-field sizes the library builds, for this cell, the slots and sorted
  * partner hits of two flips whose balls share no site (the same for every such pair); NCE_NN_FIXED_BALLS=0 in the
  * environment at this call leaves them out, and every step then builds its balls in the LDS hash table.
+ * NCE_NN_LDS_MODE_MAX=k (0..3) keeps at most the first k per-step arrays (layer-0 change, sorted hits, projections)
+ * in LDS and the rest in the scratch; every placement computes the same step (a test and diagnosis switch).
  * nce_nn_sizes: per-replica sizes (elements of the precision's type) of the cache and the scratch. */
 int nce_nn_model_create(nce_ctx *ctx, const nce_nn_model_desc *desc, int32_t *model_id);
 int nce_nn_sizes(nce_ctx *ctx, int32_t model, int64_t *cache_elems, int64_t *scratch_elems);

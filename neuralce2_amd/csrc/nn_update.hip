@@ -1398,6 +1398,14 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     }
     m.lds_mode32 = nn_lds_mode<float>(m);
     m.lds_mode64 = nn_lds_mode<double>(m);
+    // NCE_NN_LDS_MODE_MAX=k (0..3) at model creation keeps at most the first k per-step arrays in LDS (the others in
+    // the scratch): every placement computes the same step
+    if (const char *cap_env = std::getenv("NCE_NN_LDS_MODE_MAX")) {
+        const int cap = std::atoi(cap_env);
+        if (cap < 0 || cap > 3) return set_err(NCE_ERR_ARG, "NCE_NN_LDS_MODE_MAX must be 0..3");
+        m.lds_mode32 = std::min<int32_t>(m.lds_mode32, cap);
+        m.lds_mode64 = std::min<int32_t>(m.lds_mode64, cap);
+    }
     if (nn_lds_for<float>(m, 0).total > 160 * 1024 || nn_lds_for<double>(m, 0).total > 160 * 1024)
         return set_err(NCE_ERR_UNSUPPORTED, "receptive field too large for the LDS hash table");
     c->nn_models.push_back(m);

@@ -333,3 +333,46 @@ def test_fixed_balls_match_hash_path(torch_dev, ptni_nce, monkeypatch, kind, n, 
         dEs.append(dE.cpu().numpy())
     assert np.all(dEs[1] != 0.0)
     assert np.max(np.abs(dEs[0] - dEs[1])) < tol_e
+
+
+@pytest.mark.parametrize("cell", ["4", "12"])
+def test_lds_placements_agree(torch_dev, ptni_nce, monkeypatch, cell):
+    """The per-step arrays (layer-0 change, sorted hits, projections) live in LDS where they fit and in the replica's
+    scratch otherwise; NCE_NN_LDS_MODE_MAX caps how many go to LDS.  Every placement, with the gather form each one
+    selects, gives the same occupations and acceptances as the default on the same injected stream, and energies to
+    the precision's bar: FP64, FP32 and BF16, on a cell where every swap builds its balls in the hash table (256
+    sites) and one where most use the fixed tables (6,912 sites)."""
+    import torch
+    from neuralce2_amd.nce.mc_model import NNModel
+    from neuralce2_amd.nce.mcmc import NNJobState
+    mdl, cs, P, mu, sd = ptni_nce
+    at = _cell(cell, 7)
+    models = {}
+    for cap in (None, 0, 1, 2):
+        if cap is None:
+            monkeypatch.delenv("NCE_NN_LDS_MODE_MAX", raising=False)
+        else:
+            monkeypatch.setenv("NCE_NN_LDS_MODE_MAX", str(cap))
+        models[cap] = NNModel(cs, at, P, mu, sd, mdl["shell_edges"], ys=mdl["ys"], ym=mdl["ym"])
+    monkeypatch.delenv("NCE_NN_LDS_MODE_MAX", raising=False)
+    N, perm = models[None].N, models[None].geo["perm"]
+    rng = np.random.default_rng(11)
+    R, steps = 4, 100
+    sig_o = (at.numbers == 78).astype(np.int8)
+    occ_o = np.stack([rng.permutation(sig_o) for _ in range(R)])
+    betas = (1.0 / (8.617333262e-5 * np.geomspace(500, 1500, R))).astype(np.float32).astype(np.float64)
+    ia = rng.integers(0, N, (R, steps)); ib = rng.integers(0, N, (R, steps)); u = rng.random((R, steps))
+    inj = tuple(torch.tensor(x, device=torch_dev) for x in (ia, ib, u))
+    for prec in (0, 1, 2):
+        tol = 1e-10 if prec == 0 else 2e-5 / mdl["ys"]
+        res = {}
+        for cap, nn in models.items():
+            st = NNJobState(nn, occ_o[:, perm], betas, precision=prec, sample_interval=1, samples_per_chunk=steps)
+            st.run(steps, inj)
+            torch.cuda.synchronize()
+            res[cap] = (st.occ.cpu().numpy(), st.n_accept.cpu().numpy(), st.energy.cpu().numpy())
+        ref = res[None]
+        for cap in (0, 1, 2):
+            assert np.array_equal(res[cap][0], ref[0]), (prec, cap)
+            assert np.array_equal(res[cap][1], ref[1]), (prec, cap)
+            assert np.max(np.abs(res[cap][2] - ref[2])) < tol, (prec, cap)
