@@ -208,9 +208,15 @@ __global__ void nn_energy_finish_kernel(const NNModelDev *__restrict__ models, i
 constexpr int NN_BS = 768;
 constexpr int NN_EMPTY = -1;
 constexpr int NN_RB = 4;                 // rows per thread in the projection
+constexpr int NN_LA = 8;                 // proposals drawn, and their species read, ahead of the steps (power of 2)
+// one drawn proposal and the species of its two sites at the step it runs
+struct NNProp {
+    double u;
+    int ia, ib, si, sj;
+};
 
 struct NNLds {
-    size_t keys, vals, slot_site, cnt, red, misc, wv, part, w, dh0, list, pr, total;
+    size_t keys, vals, slot_site, cnt, red, misc, la, wv, part, w, dh0, list, pr, total;
 };
 __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int win_n, int n_part, int n_c0, size_t w_bytes,
                                                size_t dh0_bytes, size_t list_bytes, size_t pr_bytes) {
@@ -222,6 +228,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
     L.cnt = o;       o = align16(o + (size_t)(ball_cap + 1) * sizeof(int));
     L.red = o;       o = align16(o + (NN_BS / 64 + 1) * sizeof(double));
     L.misc = o;      o = align16(o + (4 + NN_MAXL + 1) * sizeof(int));
+    L.la = o;        o = align16(o + NN_LA * sizeof(NNProp));
     L.wv = o;        o = align16(o + (size_t)win_n * 3 * sizeof(int));
     L.part = o;      o = align16(o + (size_t)n_part * (sizeof(int4) + sizeof(int)) + (size_t)n_c0 * sizeof(int));
     L.w = o;         o = align16(o + w_bytes);
@@ -421,16 +428,32 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     bool w_staged = false;                             // the projection matrix in LDS is the current layer's
     bool pre_in_list = false;                          // list and cnt hold the fixed table (one-layer models)
 
+    NNProp *s_la = (NNProp *)(smem + lay.la);
     for (int64_t t = 0; t < n_steps; ++t) {
-        int64_t ia, ib;
-        double u;
-        if (job.inj_ia) {
-            ia = gptr(job.inj_ia)[inj_base + t]; ib = gptr(job.inj_ib)[inj_base + t]; u = gptr(job.inj_u)[inj_base + t];
-        } else {
-            const Proposal p = draw_proposal_1b(job.seed, step0 + (uint64_t)t, (uint32_t)r, (uint64_t)m.n_cells);
-            ia = p.ia; ib = p.ib; u = p.u;
+        const int la_slot = (int)(t & (NN_LA - 1));
+        if (la_slot == 0) {
+            // the next NN_LA proposals, one lane each, with the species of their sites read together (a swap
+            // accepted before a proposal's step patches them below)
+            if (tid < NN_LA && t + tid < n_steps) {
+                int64_t pa, pb;
+                double pu;
+                if (job.inj_ia) {
+                    pa = gptr(job.inj_ia)[inj_base + t + tid];
+                    pb = gptr(job.inj_ib)[inj_base + t + tid];
+                    pu = gptr(job.inj_u)[inj_base + t + tid];
+                } else {
+                    const Proposal p =
+                        draw_proposal_1b(job.seed, step0 + (uint64_t)(t + tid), (uint32_t)r, (uint64_t)m.n_cells);
+                    pa = p.ia; pb = p.ib; pu = p.u;
+                }
+                s_la[tid] = NNProp{pu, (int)pa, (int)pb, (int)occ[pa], (int)occ[pb]};
+            }
+            __syncthreads();
         }
-        const int si = occ[ia], sj = occ[ib];
+        const NNProp pq = s_la[la_slot];
+        const int64_t ia = pq.ia, ib = pq.ib;
+        const double u = pq.u;
+        const int si = pq.si, sj = pq.sj;
         if (ia == ib || si == sj) {                      // no-op proposal: dE = 0, accepted (reference semantics)
             if (eval_dE) {
                 if (tid == 0) eval_dE[blockIdx.x] = 0.0;
@@ -1090,6 +1113,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 if (tid == 0) {
                     occ[ia] = (int8_t)sj;
                     occ[ib] = (int8_t)si;
+                }
+                // later proposals of the batch see the swap: occ[ia] is now sj, occ[ib] is si
+                if (tid > la_slot && tid < NN_LA) {
+                    NNProp &q = s_la[tid];
+                    q.si = q.ia == ia ? sj : q.ia == ib ? si : q.si;
+                    q.sj = q.ib == ia ? sj : q.ib == ib ? si : q.sj;
                 }
                 E += dE;
                 ++nacc;
