@@ -205,7 +205,7 @@ __global__ void nn_energy_finish_kernel(const NNModelDev *__restrict__ models, i
 }
 
 // ── local update chunk kernel ─────────────────────────────────────────────────────────────────────────────
-constexpr int NN_BS = 768;
+constexpr int NN_BS = 1024;                // threads per replica: 16 waves, 4 per SIMD
 constexpr int NN_EMPTY = -1;
 constexpr int NN_RB = 4;                 // rows per thread in the projection
 constexpr int NN_LA = 8;                 // proposals drawn, and their species read, ahead of the steps (power of 2)
