@@ -36,7 +36,7 @@ PUBLISHED_H100 = 4_605_830.0      # reference: CE attempts/s at N = 1,048,576, R
 # HBM bytes per swap attempt moved by the chunk kernel, from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per
 # dispatch (factors from a 1 GiB device copy): profiles/ce_chunk_spec_kernel_fcc64cubic_1m_r256.md and
 # profiles/nn_chunk_kernel_ptni_fcc16cubic_r32.md
-MEASURED_TRAFFIC_PER_ATTEMPT = {"ce_1m": 3461.0, "nce_ptni_16k": 80980.0}
+MEASURED_TRAFFIC_PER_ATTEMPT = {"ce_1m": 3461.0, "nce_ptni_16k": 82177.0}
 
 CONFIGS = {
     "ce_1m": dict(n=64, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
