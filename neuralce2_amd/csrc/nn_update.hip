@@ -124,17 +124,22 @@ __global__ __launch_bounds__(NN_H0_SITES) void nn_h0_kernel(const NNModelDev *__
     }
 }
 
-// pre-activations of hidden layer l at every site (inputs: h0s for l = 0, else the activations of layer l - 1)
+constexpr int NN_LDC = 32;               // output channels per thread in the full recompute of a layer
+
+// pre-activations of hidden layer l at every site (inputs: h0s for l = 0, else the activations of layer l - 1).
+// One site and NN_LDC output channels per thread (the channel block is the same for the whole workgroup, so the
+// weights are read as broadcasts): the previous layer's activations and the shell means are formed once per block
+// of channels, and every output channel sums its terms in the order of the layer formula.
 template <typename Real>
 __global__ __launch_bounds__(256) void nn_layer_kernel(const NNModelDev *__restrict__ models, int model, int l,
                                                        int64_t N, Real *__restrict__ cache_all) {
     const NNModelDev &m = models[model];
     const int r = blockIdx.y;
     const int D = m.dd[l], C = m.cin[l];
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= N * D) return;
-    const int64_t site = idx / D;
-    const int d = (int)(idx - site * D);
+    const int nch = (D + NN_LDC - 1) / NN_LDC;
+    const int d0 = (int)(blockIdx.x % nch) * NN_LDC;
+    const int64_t site = (int64_t)(blockIdx.x / nch) * 256 + threadIdx.x;
+    if (site >= N) return;
     Real *cache = cache_all + (int64_t)r * m.cache_elems;
     const Real *W = nn_w<Real>(m);
     const Real *ip = cache + (l == 0 ? 0 : m.off_pre[l - 1]);
@@ -143,9 +148,20 @@ __global__ __launch_bounds__(256) void nn_layer_kernel(const NNModelDev *__restr
         if (l == 0) return ip[x * C + c];
         return nn_act<Real>(m, ip[x * C + c], ig ? ig[x * C + c] : (Real)0);
     };
-    Real pre = W[m.off_b[l] + d];
-    Real gpre = m.has_gate ? W[m.off_gb[l] + d] : (Real)0;
-    for (int c = 0; c < C; ++c) pre += get(site, c) * W[m.off_self[l] + (int64_t)c * D + d];
+    Real pre[NN_LDC], gpre[NN_LDC];
+#pragma unroll
+    for (int j = 0; j < NN_LDC; ++j) {
+        const int d = d0 + j;
+        pre[j] = d < D ? W[m.off_b[l] + d] : (Real)0;
+        gpre[j] = m.has_gate && d < D ? W[m.off_gb[l] + d] : (Real)0;
+    }
+    for (int c = 0; c < C; ++c) {
+        const Real hs = get(site, c);
+        const Real *ws = W + m.off_self[l] + (int64_t)c * D + d0;
+#pragma unroll
+        for (int j = 0; j < NN_LDC; ++j)
+            if (d0 + j < D) pre[j] += hs * ws[j];
+    }
     int q0 = 0;
     for (int p = 0; p < m.P; ++p) {
         const int np_ = m.shell_count[p];
@@ -154,13 +170,23 @@ __global__ __launch_bounds__(256) void nn_layer_kernel(const NNModelDev *__restr
             Real z = (Real)0;
             for (int q = 0; q < np_; ++q) z += get(nn_site(m, site, m.part[q0 + q]), c);
             z *= inv;
-            pre += z * W[m.off_shell[l] + ((int64_t)p * C + c) * D + d];
-            if (m.has_gate) gpre += z * W[m.off_gate[l] + ((int64_t)p * C + c) * D + d];
+            const Real *wsh = W + m.off_shell[l] + ((int64_t)p * C + c) * D + d0;
+            const Real *wg = W + m.off_gate[l] + ((int64_t)p * C + c) * D + d0;
+#pragma unroll
+            for (int j = 0; j < NN_LDC; ++j) {
+                if (d0 + j >= D) continue;
+                pre[j] += z * wsh[j];
+                if (m.has_gate) gpre[j] += z * wg[j];
+            }
         }
         q0 += np_;
     }
-    cache[m.off_pre[l] + site * D + d] = pre;
-    if (m.has_gate) cache[m.off_gpre[l] + site * D + d] = gpre;
+#pragma unroll
+    for (int j = 0; j < NN_LDC; ++j) {
+        if (d0 + j >= D) continue;
+        cache[m.off_pre[l] + site * D + d0 + j] = pre[j];
+        if (m.has_gate) cache[m.off_gpre[l] + site * D + d0 + j] = gpre[j];
+    }
 }
 
 // site energies and per-block partial sums in fixed order
@@ -1462,9 +1488,8 @@ static int nn_full_cache_impl(nce_ctx *c, int32_t model, const int8_t *occ, int3
     hipLaunchKernelGGL(nn_h0_kernel<Real>, dim3(nb0, R), dim3(NN_H0_SITES), lds0, s, dm, model, occ, N, cache);
     HIP_TRY(hipGetLastError());
     for (int l = 0; l < m.L; ++l) {
-        const int64_t n = N * m.dd[l];
-        hipLaunchKernelGGL(nn_layer_kernel<Real>, dim3((unsigned)((n + 255) / 256), R), dim3(256), 0, s, dm, model,
-                           l, N, cache);
+        const int64_t nblk = ((N + 255) / 256) * ((m.dd[l] + NN_LDC - 1) / NN_LDC);
+        hipLaunchKernelGGL(nn_layer_kernel<Real>, dim3((unsigned)nblk, R), dim3(256), 0, s, dm, model, l, N, cache);
         HIP_TRY(hipGetLastError());
     }
     const int nbe = (int)((N + 255) / 256);
