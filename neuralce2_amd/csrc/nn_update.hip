@@ -94,6 +94,13 @@ __device__ inline bool nn_balls_apart(const NNModelDev &m, int64_t ia, int64_t i
 // ── full cache ─────────────────────────────────────────────────────────────────────────────────────────────
 constexpr int NN_H0_SITES = 64;
 
+// LDS bytes per site of the window-sign rows: an odd number of 4-byte words, so the rows start in different banks;
+// 0 (members read from memory) where the rows and the channel accumulators would pass 96 KB per workgroup
+__host__ __device__ inline int nn_h0_sign_stride(int W0, int C0, int real_bytes) {
+    const int stride = 4 * (((W0 + 3) / 4) | 1);
+    return (size_t)NN_H0_SITES * ((size_t)C0 * real_bytes + stride) <= 96 * 1024 ? stride : 0;
+}
+
 template <typename Real>
 __global__ __launch_bounds__(NN_H0_SITES) void nn_h0_kernel(const NNModelDev *__restrict__ models, int model,
                                                             const int8_t *__restrict__ occ_all, int64_t N,
@@ -106,15 +113,24 @@ __global__ __launch_bounds__(NN_H0_SITES) void nn_h0_kernel(const NNModelDev *__
     const int8_t *occ = occ_all + (int64_t)r * N;
     Real *row = acc + (size_t)threadIdx.x * m.C0;
     for (int c = 0; c < m.C0; ++c) row[c] = (Real)0;
+    // the spin (+1 / -1) of every window slot of this thread's site, read once: [NN_H0_SITES][stride] bytes
+    const int stride = nn_h0_sign_stride(m.W0, m.C0, (int)sizeof(Real));
+    int8_t *sg = (int8_t *)(acc + (size_t)NN_H0_SITES * m.C0) + (size_t)threadIdx.x * stride;
     if (site < N) {
+        if (stride > 0)
+            for (int k = 0; k < m.W0; ++k) sg[k] = occ[nn_site(m, site, m.win[k])] ? 1 : -1;
         for (int e = 0; e < m.J0; ++e) {
             const uint2 sl = m.ent_slot[e];
             const uint32_t s4[4] = {sl.x & 0xffffu, sl.x >> 16, sl.y & 0xffffu, sl.y >> 16};
             double prod = 1.0;
             for (int q = 0; q < 4; ++q) {
                 if (s4[q] == 0xffffu) continue;
-                const int64_t x = nn_site(m, site, m.win[s4[q]]);
-                prod *= occ[x] ? 1.0 : -1.0;
+                if (stride > 0) {
+                    prod *= (double)sg[s4[q]];
+                } else {
+                    const int64_t x = nn_site(m, site, m.win[s4[q]]);
+                    prod *= occ[x] ? 1.0 : -1.0;
+                }
             }
             row[m.ent_col[e]] += (Real)(prod * m.ent_coef[e]);
         }
@@ -1482,7 +1498,7 @@ static int nn_full_cache_impl(nce_ctx *c, int32_t model, const int8_t *occ, int3
     const NNModelDev &m = c->nn_models[model];
     const NNModelDev *dm = (const NNModelDev *)c->d_nn_models;
     const int nb0 = (int)((N + NN_H0_SITES - 1) / NN_H0_SITES);
-    const size_t lds0 = (size_t)NN_H0_SITES * m.C0 * sizeof(Real);
+    const size_t lds0 = (size_t)NN_H0_SITES * (m.C0 * sizeof(Real) + nn_h0_sign_stride(m.W0, m.C0, (int)sizeof(Real)));
     if (lds0 > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "too many layer-0 channels");
     if (int rc = allow_lds(nn_h0_kernel<Real>, lds0)) return rc;
     hipLaunchKernelGGL(nn_h0_kernel<Real>, dim3(nb0, R), dim3(NN_H0_SITES), lds0, s, dm, model, occ, N, cache);
