@@ -316,6 +316,8 @@ __device__ inline int nn_hash_find(const int *keys, const int *vals, int mask, i
     }
 }
 
+// block-wide sum, the same in every thread.  red is read after the function returns, up to the caller's next
+// barrier: nothing may write red before that barrier (the chunk kernel's next write is steps of barriers later)
 template <typename Real>
 __device__ inline Real nn_block_sum(Real v, Real *red) {
     v = wave_sum(v);
@@ -324,7 +326,6 @@ __device__ inline Real nn_block_sum(Real v, Real *red) {
     __syncthreads();
     Real s = red[0];
     for (int i = 1; i < NN_BS / 64; ++i) s += red[i];
-    __syncthreads();
     return s;
 }
 
