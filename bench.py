@@ -234,6 +234,11 @@ def main():
                                        f"{args.chunk}"},
                 "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"], "wall_s": st["wall_s"],
                 "attempts": st["attempts"], "roofline": None, "cpu_baseline": None}
+        for kind in ("ce", "nce"):
+            lat = np.array([r.latency for r in res if r.kind == kind])
+            if len(lat):
+                line[f"latency_{kind}_p50_s"] = float(np.percentile(lat, 50))
+                line[f"latency_{kind}_p99_s"] = float(np.percentile(lat, 99))
         print(json.dumps(line), flush=True)
         return
     if args.served:

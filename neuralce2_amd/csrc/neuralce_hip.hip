@@ -168,8 +168,15 @@ struct nce_ctx {
         hipEvent_t ev = nullptr;
         bool pending = false;
     };
-    Stage stage[4];
-    int next_stage = 0;
+    // Staged uploads of job tables: one ring per stream.  A slot's next copy is ordered after the kernels that read
+    // it by the stream itself; the slot's event (after its copy) guards the pinned host buffer.  Rings shared by
+    // streams would let a copy on one stream overwrite a table that a long launch on another still reads.
+    struct StageRing {
+        hipStream_t stream = nullptr;
+        Stage stage[4];
+        int next = 0;
+    };
+    std::vector<StageRing *> rings;
     double *d_partial = nullptr;
     size_t partial_cap = 0;
     std::vector<NNModelDev> nn_models;
@@ -1311,12 +1318,6 @@ extern "C" int nce_ctx_create(int32_t device_ordinal, nce_ctx **out) {
     c->device = device_ordinal;
     c->n_cu = prop.multiProcessorCount;
     c->max_lds = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
-    for (auto &s : c->stage) {
-        if (hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess) {
-            delete c;
-            return set_err(NCE_ERR_HIP, "hipEventCreate failed");
-        }
-    }
     *out = c;
     return NCE_OK;
 }
@@ -1329,10 +1330,13 @@ extern "C" void nce_ctx_destroy(nce_ctx *c) {
     if (c->d_models) (void)hipFree(c->d_models);
     if (c->d_partial) (void)hipFree(c->d_partial);
     if (c->d_nn_models) (void)hipFree(c->d_nn_models);
-    for (auto &s : c->stage) {
-        if (s.host) (void)hipHostFree(s.host);
-        if (s.dev) (void)hipFree(s.dev);
-        if (s.ev) (void)hipEventDestroy(s.ev);
+    for (nce_ctx::StageRing *ring : c->rings) {
+        for (auto &st : ring->stage) {
+            if (st.host) (void)hipHostFree(st.host);
+            if (st.dev) (void)hipFree(st.dev);
+            if (st.ev) (void)hipEventDestroy(st.ev);
+        }
+        delete ring;
     }
     delete c;
 }
@@ -1618,8 +1622,18 @@ extern "C" int nce_ce_delta(nce_ctx *c, int32_t model, const int8_t *occ, int32_
 
 // stage a host buffer into device memory, stream-ordered; a ring of pinned buffers guarded by events
 static int stage_upload(nce_ctx *c, const void *host, size_t bytes, hipStream_t s, void **dev) {
-    nce_ctx::Stage &st = c->stage[c->next_stage];
-    c->next_stage = (c->next_stage + 1) % 4;
+    nce_ctx::StageRing *ring = nullptr;
+    for (nce_ctx::StageRing *g : c->rings)
+        if (g->stream == s) ring = g;
+    if (!ring) {
+        ring = new nce_ctx::StageRing();
+        ring->stream = s;
+        c->rings.push_back(ring);
+        for (auto &st : ring->stage)
+            HIP_TRY(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming));
+    }
+    nce_ctx::Stage &st = ring->stage[ring->next];
+    ring->next = (ring->next + 1) % 4;
     if (st.pending) {
         HIP_TRY(hipEventSynchronize(st.ev));
         st.pending = false;
