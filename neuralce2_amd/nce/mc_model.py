@@ -10,7 +10,7 @@ import numpy as np
 
 from ..clusterspace import ClusterSpace
 from .gpu_layer0_implicit import build_geometry
-from .model_io import expand_params, load_model  # noqa: F401  (re-exported like the reference module)
+from .model_io import check_supported, ensemble_params, expand_params, load_model  # noqa: F401
 
 
 def silu_network(c0, width, depth, n_shells, seed=0, glorot=True):
@@ -38,6 +38,7 @@ class NNModel:
         from .._lib import Context
         if not torch.cuda.is_available():
             raise RuntimeError("the NeuralCE MC path needs a ROCm GPU (libneuralce_hip.so); none is visible")
+        check_supported(P)
         self.cs = cs
         self.P, self.mu, self.sd = P, mu, sd
         self.depth = len(P["layers"])
@@ -76,11 +77,14 @@ class NNModel:
 
 
 def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0):
-    """Build the MC machinery for `atoms` (an ideal-lattice supercell of the model's primitive); see the module doc."""
-    if seed_idx is not None and int(seed_idx) < 0:
-        raise NotImplementedError("the seed-ensemble model is not part of the gfx950 path; pick one seed")
+    """Build the MC machinery for `atoms` (an ideal-lattice supercell of the model's primitive); see the module doc.
+    seed_idx < 0 selects the seed ensemble (mean energy over every saved parameter set), run as one network whose
+    width is the sum of the seeds' widths (model_io.ensemble_params)."""
     cs = ClusterSpace(mdl["primitive"], list(mdl["cutoffs"]), mdl["chemical_symbols"])
-    P, mu, sd = expand_params(mdl, seed_idx)
+    if seed_idx is not None and int(seed_idx) < 0:
+        P, mu, sd = ensemble_params(mdl)
+    else:
+        P, mu, sd = expand_params(mdl, 0 if seed_idx is None else int(seed_idx))
     nn = NNModel(cs, atoms, P, mu, sd, list(mdl["shell_edges"]), act="gate", readout="mean",
                  ys=mdl["ys"], ym=mdl["ym"], device=device)
     prec = {"float64": 0, None: 0, "bfloat16": 2, "bf16": 2}.get(dtype, 1)

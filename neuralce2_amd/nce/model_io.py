@@ -73,3 +73,62 @@ def expand_params(mdl, seed_idx=0):
     mu[keep] = np.asarray(mdl["mu"], float); sd[keep] = np.asarray(mdl["sd"], float)
     head = {k: np.asarray(v, float) for k, v in P["head"].items()}
     return dict(lin=lin, layers=layers, head=head), mu, sd
+
+
+def ensemble_params(mdl):
+    """The seed ensemble (reference mc_model.py:175-190, ``seed_idx < 0``: energy = mean over the saved parameter
+    sets) as ONE network of width K D, for K seeds of width D.
+
+    Every operation of a layer is either a linear map or elementwise per output channel (activation, gate, shell
+    mean), so K networks side by side are one network whose first layer concatenates the K maps along the output
+    channels and whose deeper layers are block-diagonal.  The readout ``h0 . lin + h_L . w + b`` is linear in the
+    per-seed terms, so the mean over seeds takes lin = mean(lin_k), w = concat(w_k) / K and b = mean(b_k).  The
+    result equals the mean of the K per-seed energies and energy changes exactly (up to rounding)."""
+    sets = [expand_params(mdl, s) for s in range(len(mdl["params"]))]
+    K = len(sets)
+    Ps = [s[0] for s in sets]
+    mu, sd = sets[0][1], sets[0][2]
+    depth = len(Ps[0]["layers"])
+    layers = []
+    for l in range(depth):
+        Ls = [P["layers"][l] for P in Ps]
+        if any(set(L) != set(Ls[0]) for L in Ls):
+            raise ValueError("the saved parameter sets differ in their layer structure; no ensemble")
+        out = {}
+        for key in Ls[0]:
+            parts = [np.asarray(L[key], float) for L in Ls]
+            if key in ("b", "gb"):
+                out[key] = np.concatenate(parts)
+            elif l == 0:                               # (C0, D) or (P, C0, D): concatenate along D
+                out[key] = np.concatenate(parts, axis=-1)
+            else:                                      # (D, D) or (P, D, D): block-diagonal over the seeds
+                cin, d = parts[0].shape[-2], parts[0].shape[-1]
+                lead = parts[0].shape[:-2]
+                M = np.zeros(lead + (K * cin, K * d))
+                for k, A in enumerate(parts):
+                    M[..., k * cin:(k + 1) * cin, k * d:(k + 1) * d] = A
+                out[key] = M
+        layers.append(out)
+    lin = np.mean([P["lin"] for P in Ps], axis=0)
+    head = dict(b=np.asarray(np.mean([float(P["head"]["b"]) for P in Ps])))
+    if depth:
+        head["w"] = np.concatenate([np.asarray(P["head"]["w"], float) for P in Ps]) / K
+    return dict(lin=lin, layers=layers, head=head), mu, sd
+
+
+def check_supported(P):
+    """Refuse parameter sets the gfx950 kernels do not evaluate, rather than run them with a term missing: the
+    max-over-shell branch (``shell_max``, reference local_update_implicit.py:49-51,69-73) and a gate present in some
+    layers but not in all."""
+    layers = P["layers"]
+    for l, L in enumerate(layers):
+        if "shell_max" in L:
+            raise NotImplementedError(f"layer {l} has a max-over-shell branch (shell_max); the gfx950 local update "
+                                      f"evaluates mean-over-shell layers only")
+    gated = ["gate" in L for L in layers]
+    if any(gated) and not all(gated):
+        raise NotImplementedError(f"gate maps in layers {[l for l, g in enumerate(gated) if g]} only; the kernels "
+                                  f"take a gate in every layer or in none")
+    for l, L in enumerate(layers):
+        if ("gate" in L) != ("gb" in L):
+            raise ValueError(f"layer {l}: gate maps and gate bias must come together")

@@ -21,18 +21,28 @@ from __future__ import annotations
 import numpy as np
 
 
+_GRID = 1 << 20          # fractional coordinates of the supercell quantised to 2^-20 for the site key
+
+
+def _keys(f):
+    g = np.mod(np.rint(f * _GRID).astype(np.int64), _GRID)
+    return (g[:, 0] * _GRID + g[:, 1]) * _GRID + g[:, 2], np.abs(f * _GRID - np.rint(f * _GRID)).max(-1)
+
+
 def _site_lookup(atoms):
+    """xyz (M, 3) -> index of the supercell site at that position, modulo the supercell: sites are keyed by their
+    wrapped fractional coordinates on a 2^-20 grid and found by binary search (O(M log N))."""
     Sinv = np.linalg.inv(atoms.cell)
-    fs = atoms.get_scaled_positions()
+    k_site, _ = _keys(np.mod(atoms.positions @ Sinv, 1.0))
+    order = np.argsort(k_site, kind="stable")
+    k_sorted = k_site[order]
+    assert len(np.unique(k_sorted)) == len(k_sorted), "two sites share a position"
 
     def site_at(xyz):
-        f = np.mod(np.asarray(xyz) @ Sinv, 1.0)
-        dd = f[:, None, :] - fs[None, :, :]
-        dd -= np.round(dd)
-        err = np.abs(dd).max(-1)
-        j = err.argmin(1)
-        assert np.all(err[np.arange(len(j)), j] < 1e-5), "position not on the supercell lattice"
-        return j
+        k, off = _keys(np.mod(np.asarray(xyz, float).reshape(-1, 3) @ Sinv, 1.0))
+        pos = np.minimum(np.searchsorted(k_sorted, k), len(k_sorted) - 1)
+        assert np.all(k_sorted[pos] == k) and np.all(off < 20.0), "position not on the supercell lattice"
+        return order[pos]
     return site_at
 
 
