@@ -86,6 +86,12 @@ typedef struct {
     const int64_t *inj_ia;      /* [R][n_steps] injected proposals (site a), or NULL for the Philox stream */
     const int64_t *inj_ib;      /* [R][n_steps] */
     const double *inj_u;        /* [R][n_steps] Metropolis uniforms */
+    double *acc_n;              /* [R] by temperature slot, or NULL.  With accumulate: after the chunk, each replica's
+                                   (s1, s2) around e_ref are folded into the FP64 moments of its slot (count, mean,
+                                   sum of squared deviations; Chan's parallel merge), then s1 = s2 = 0 and
+                                   e_ref = energy, on the device */
+    double *acc_mean;           /* [R] */
+    double *acc_m2;             /* [R] */
 } nce_pt_job;
 
 /* Temperature-swap exchange of one job (device pointers). */
@@ -103,7 +109,11 @@ typedef struct {
     int64_t *n_swaps;           /* [1] accumulated accepted exchanges */
     const double *inj_u;        /* [ceil((R-1)/2)] injected uniforms or NULL */
     double *traj_last;          /* [R] if not NULL: rewritten as energy[rep_of_slot[t]] after the exchange */
+    int64_t *ex_count;          /* [1] or NULL.  If set, ex_index is ignored: the exchange uses *ex_count + 1 and
+                                   stores it back, so a captured cycle draws a new parity and uniforms each replay */
 } nce_pt_exchange_job;
+
+typedef struct nce_pt_cycle nce_pt_cycle;
 
 const char *nce_last_error(void);
 
@@ -141,6 +151,20 @@ int nce_pt_run_chunk(nce_ctx *ctx, const nce_pt_job *jobs, int32_t n_jobs, int64
 /* Temperature-swap exchange of each job; replaces exchange_slots (neuralce/ce/pt_mcmc_ce.py:748-760) and the
  * reordering of the last sample after it (:808-810).  jobs: HOST array. */
 int nce_pt_exchange(nce_ctx *ctx, const nce_pt_exchange_job *jobs, int32_t n_jobs, void *stream);
+
+/* One exchange cycle of a fixed set of jobs captured as a HIP graph: chunks_per_cycle chunks of n_steps (each
+ * followed by the moment merge of the jobs that give acc_n), then the exchange of every job in ex_jobs.  Replaces
+ * the jitted run_chunk, whose scan runs chunk_size steps and fires exchange_slots at the exchange interval
+ * (neuralce/ce/pt_mcmc_ce.py:798-818): each nce_pt_cycle_launch replays the whole cycle n_cycles times with one
+ * graph launch each and no host work in between.  The job and exchange tables are copied into device memory owned
+ * by the cycle at creation (the caller's buffers they point to must outlive it); the exchange jobs must give
+ * ex_count; injected streams are refused.  Launch parameters (kernel, tile, LDS) are fixed at creation as
+ * nce_pt_run_chunk would choose them.  A cycle belongs to its context's device; destroy it before the context. */
+int nce_pt_cycle_create(nce_ctx *ctx, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps,
+                        int32_t chunks_per_cycle, int32_t precision, int32_t tile,
+                        const nce_pt_exchange_job *ex_jobs, int32_t n_ex_jobs, nce_pt_cycle **out);
+int nce_pt_cycle_launch(nce_pt_cycle *cycle, int32_t n_cycles, void *stream);
+void nce_pt_cycle_destroy(nce_pt_cycle *cycle);
 
 /* ── Neural cluster expansion (receptive-field local update) ─────────────────────────────────────────────────
  * Model on one cell shape: layer-0 template (window of the sites sharing a cluster with a site, reverse entries with

@@ -35,13 +35,13 @@ class PTJob(C.Structure):
                 ("energy", _p), ("beta", _p), ("slot_of_rep", _p), ("step", _p), ("seed", C.c_uint64),
                 ("n_accept", _p), ("e_min", _p), ("traj", _p), ("sample_interval", C.c_int32),
                 ("accumulate", C.c_int32), ("e_ref", _p), ("s1", _p), ("s2", _p), ("inj_ia", _p), ("inj_ib", _p),
-                ("inj_u", _p)]
+                ("inj_u", _p), ("acc_n", _p), ("acc_mean", _p), ("acc_m2", _p)]
 
 
 class ExchangeJob(C.Structure):
     _fields_ = [("n_replicas", C.c_int32), ("inj_parity", C.c_int32), ("energy", _p), ("rep_of_slot", _p),
                 ("slot_of_rep", _p), ("beta_slot", _p), ("ladder_id", _p), ("beta_rep", _p), ("seed", C.c_uint64),
-                ("ex_index", C.c_uint64), ("n_swaps", _p), ("inj_u", _p), ("traj_last", _p)]
+                ("ex_index", C.c_uint64), ("n_swaps", _p), ("inj_u", _p), ("traj_last", _p), ("ex_count", _p)]
 
 
 class NNModelDesc(C.Structure):
@@ -75,6 +75,11 @@ def _load():
     lib.nce_ce_delta.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p, _p, C.c_int32, _p]
     lib.nce_pt_run_chunk.argtypes = [_p, _p, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _p]
     lib.nce_pt_exchange.argtypes = [_p, _p, C.c_int32, _p]
+    lib.nce_pt_cycle_create.argtypes = [_p, _p, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_int32, _p, C.c_int32,
+                                        C.POINTER(_p)]
+    lib.nce_pt_cycle_launch.argtypes = [_p, C.c_int32, _p]
+    lib.nce_pt_cycle_destroy.argtypes = [_p]
+    lib.nce_pt_cycle_destroy.restype = None
     lib.nce_nn_model_create.argtypes = [_p, C.POINTER(NNModelDesc), C.POINTER(C.c_int32)]
     lib.nce_nn_sizes.argtypes = [_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.nce_nn_full_cache.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p, C.c_int32, _p]
@@ -88,8 +93,9 @@ def _load():
 lib = _load()
 
 EXPORTED = ("nce_last_error", "nce_ctx_create", "nce_ctx_destroy", "nce_ce_model_create", "nce_ce_energy",
-            "nce_ce_delta", "nce_pt_run_chunk", "nce_pt_exchange", "nce_philox4x32_10", "nce_nn_model_create",
-            "nce_nn_sizes", "nce_nn_full_cache", "nce_nn_run_chunk", "nce_nn_delta")
+            "nce_ce_delta", "nce_pt_run_chunk", "nce_pt_exchange", "nce_pt_cycle_create", "nce_pt_cycle_launch",
+            "nce_pt_cycle_destroy", "nce_philox4x32_10", "nce_nn_model_create", "nce_nn_sizes", "nce_nn_full_cache",
+            "nce_nn_run_chunk", "nce_nn_delta")
 
 
 class NCEError(RuntimeError):
@@ -197,6 +203,10 @@ class Context:
         arr = (ExchangeJob * len(jobs))(*jobs)
         check(lib.nce_pt_exchange(self.h, C.addressof(arr), len(jobs), stream_ptr(stream)))
 
+    def pt_cycle(self, jobs, n_steps, chunks_per_cycle, precision, tile, ex_jobs):
+        """capture `chunks_per_cycle` chunks (+ moment merges) and the exchange of `ex_jobs` as one HIP graph."""
+        return Cycle(self, jobs, n_steps, chunks_per_cycle, precision, tile, ex_jobs)
+
     # ── NeuralCE ──
     def nn_model_create(self, geo, P, mu, sd, act="gate", readout="mean", de_scale=1.0):
         """upload a NeuralCE model: geometry dict (gpu_layer0_implicit.build_geometry) and expanded parameters."""
@@ -251,3 +261,30 @@ class Context:
         """energy change of the injected proposal of every replica, nothing committed (dE: device float64)."""
         arr = (NNJob * len(jobs))(*jobs)
         check(lib.nce_nn_delta(self.h, C.addressof(arr), len(jobs), ptr(dE), int(precision), stream_ptr(stream)))
+
+
+class Cycle:
+    """A captured exchange cycle (nce_pt_cycle): the job tables live on the device, a launch replays the graph."""
+
+    def __init__(self, ctx, jobs, n_steps, chunks_per_cycle, precision, tile, ex_jobs):
+        arr = (PTJob * len(jobs))(*jobs)
+        ex = (ExchangeJob * max(len(ex_jobs), 1))(*ex_jobs)
+        h = _p()
+        check(lib.nce_pt_cycle_create(ctx.h, C.addressof(arr), len(jobs), int(n_steps), int(chunks_per_cycle),
+                                      int(precision), int(tile), C.addressof(ex), len(ex_jobs), C.byref(h)))
+        self.h = h
+        self.ctx = ctx                     # destroyed before the context
+
+    def launch(self, n_cycles=1, stream=None):
+        check(lib.nce_pt_cycle_launch(self.h, int(n_cycles), stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.nce_pt_cycle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

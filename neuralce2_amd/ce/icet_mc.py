@@ -20,7 +20,7 @@ from pathlib import Path
 import numpy as np
 
 from ..ce_io import ClusterExpansion
-from ..lattice import Structure, read_cif, site_decompose
+from ..lattice import OffLatticeError, Structure, read_cif, site_decompose
 
 
 def _rebuild_as_clean_supercell(atoms, primitive, cell_atol=0.1):
@@ -80,7 +80,7 @@ def _prepare_cell(ce, init_cif, supercell_reps, seed):
     else:
         try:
             site_decompose(prim, atoms)
-        except AssertionError:
+        except OffLatticeError:
             atoms = _rebuild_as_clean_supercell(atoms, prim)
     return atoms, energy_scale
 

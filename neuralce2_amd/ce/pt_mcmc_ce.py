@@ -3,8 +3,9 @@
 Drop-in for neuralce/ce/pt_mcmc_ce.py: ``MCMCConfig`` (:37-104), ``make_temperature_ladder`` (:223-229),
 ``_build_interleaved_ladders`` (:232-260), ``MCMCResult`` (:587-598) and ``run_pt_mcmc`` (:601-1028) with the same
 chunk / exchange arithmetic, the same checkpoint keys and the same output files.  The step loop is one
-``nce_pt_run_chunk`` launch per chunk (every replica of the job, every step of the chunk); the temperature-swap
-exchange is ``nce_pt_exchange`` at chunk boundaries.
+``nce_pt_run_chunk`` launch per chunk (every replica of the job, every step of the chunk, then the FP64 moment
+merge on the device); the temperature-swap exchange is ``nce_pt_exchange`` at chunk boundaries, its counter kept on
+the device.
 
 Differences from the reference, by design:
   * random streams: counter-based Philox keyed by (seed, replica, global step) instead of JAX threefry keys split per
@@ -118,7 +119,10 @@ def make_initial_state(seed, gce, initial_composition, n_replicas):
 class PTJobState:
     """Device-resident state of one PT job: occupations (R, N) int8, FP64 energies, ladder, counters, accumulators.
 
-    The same object serves the single-job driver below and the multi-job scheduler (neuralce2_amd.serve)."""
+    The per-slot FP64 moments (count, mean, sum of squared deviations of the total energy over every accumulated
+    step) and the exchange counter live on the device: a chunk launch folds its moments in and an exchange advances
+    the counter, so consecutive chunks need no host round trip.  The same object serves the single-job driver below
+    and the multi-job scheduler (neuralce2_amd.serve)."""
 
     def __init__(self, gce, sigma0, betas_slot, ladder_id=None, seed=0, sample_interval=1, samples_per_chunk=0,
                  job_id=0):
@@ -149,16 +153,20 @@ class PTJobState:
         self.e_ref = self.energy.clone()
         self.s1 = torch.zeros(R, dtype=torch.float64, device=dev)
         self.s2 = torch.zeros(R, dtype=torch.float64, device=dev)
+        self.acc_n = torch.zeros(R, dtype=torch.float64, device=dev)
+        self.acc_mean = torch.zeros(R, dtype=torch.float64, device=dev)
+        self.acc_m2 = torch.zeros(R, dtype=torch.float64, device=dev)
         self.n_swaps = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.ex_count = torch.zeros(1, dtype=torch.int64, device=dev)
         self.sample_interval = int(sample_interval)
         self.traj = torch.zeros((max(samples_per_chunk, 1), R), dtype=torch.float64, device=dev) if samples_per_chunk else None
         self.seed = int(seed) & ((1 << 64) - 1)
         self.job_id = job_id
-        # FP64 per-slot accumulators on the host (Chan's parallel merge of per-chunk moments)
-        self.acc_n = np.zeros(R, np.float64); self.acc_mean = np.zeros(R); self.acc_m2 = np.zeros(R)
         self._PTJob, self._ExchangeJob = PTJob, ExchangeJob
 
-    def job(self, accumulate=False, inj=None):
+    def job(self, accumulate=False, inj=None, merge=True):
+        """chunk-launch descriptor; accumulate adds every step's energy to the chunk moments, merge (with
+        accumulate) folds them into the per-slot accumulators on the device after the chunk."""
         from .._lib import ptr
         j = self._PTJob()
         j.model = self.gce.model; j.n_replicas = self.R; j.n_sites = self.N
@@ -168,255 +176,279 @@ class PTJobState:
         j.traj = ptr(self.traj) if self.traj is not None else None
         j.sample_interval = self.sample_interval
         j.accumulate = int(bool(accumulate)); j.e_ref = ptr(self.e_ref); j.s1 = ptr(self.s1); j.s2 = ptr(self.s2)
+        if accumulate and merge:
+            j.acc_n = ptr(self.acc_n); j.acc_mean = ptr(self.acc_mean); j.acc_m2 = ptr(self.acc_m2)
         if inj is not None:
             self._inj = inj
             j.inj_ia = ptr(inj[0]); j.inj_ib = ptr(inj[1]); j.inj_u = ptr(inj[2])
         return j
 
-    def exchange_job(self, ex_index, reorder_last=True, inj_parity=-1, inj_u=None):
+    def exchange_job(self, ex_index=None, reorder_last=True, inj_parity=-1, inj_u=None):
+        """exchange descriptor; ex_index None takes the device counter (exchange number = count + 1)."""
         from .._lib import ptr
         e = self._ExchangeJob()
         e.n_replicas = self.R; e.inj_parity = int(inj_parity)
         e.energy = ptr(self.energy); e.rep_of_slot = ptr(self.rep_of_slot); e.slot_of_rep = ptr(self.slot_of_rep)
         e.beta_slot = ptr(self.beta_slot); e.ladder_id = ptr(self.ladder_id); e.beta_rep = ptr(self.beta_rep)
-        e.seed = self.seed ^ 0x9E3779B97F4A7C15; e.ex_index = int(ex_index); e.n_swaps = ptr(self.n_swaps)
+        e.seed = self.seed ^ 0x9E3779B97F4A7C15; e.n_swaps = ptr(self.n_swaps)
+        if ex_index is None:
+            e.ex_count = ptr(self.ex_count)
+        else:
+            e.ex_index = int(ex_index)
         if inj_u is not None:
             self._inj_u = inj_u
         e.inj_u = ptr(inj_u)
         e.traj_last = ptr(self.traj[-1]) if (reorder_last and self.traj is not None) else None
         return e
 
-    def merge_accumulators(self, n_steps, slot_of_rep_np):
-        """fold the chunk's per-replica (s1, s2) around e_ref into the per-slot FP64 moments; reset for the next chunk."""
-        s1 = self.s1.cpu().numpy(); s2 = self.s2.cpu().numpy(); ref = self.e_ref.cpu().numpy()
-        n = float(n_steps)
-        mean_c = ref + s1 / n
-        m2_c = np.maximum(s2 - s1 * s1 / n, 0.0)
-        for r in range(self.R):
-            t = int(slot_of_rep_np[r])
-            na = self.acc_n[t]; nb = n; tot = na + nb
-            delta = mean_c[r] - self.acc_mean[t]
-            self.acc_mean[t] += delta * nb / tot
-            self.acc_m2[t] += m2_c[r] + delta * delta * na * nb / tot
-            self.acc_n[t] = tot
+    def reset_moments(self):
+        """start the accumulated moments afresh (end of equilibration)."""
+        self.acc_n.zero_(); self.acc_mean.zero_(); self.acc_m2.zero_()
         self.s1.zero_(); self.s2.zero_()
         self.e_ref.copy_(self.energy)
 
     def moments(self):
-        n = np.maximum(self.acc_n, 1.0)
-        return self.acc_mean.copy(), self.acc_m2 / n
+        """per temperature slot: mean and variance (population) of the total energy over the accumulated steps."""
+        n = self.acc_n.cpu().numpy()
+        return self.acc_mean.cpu().numpy(), self.acc_m2.cpu().numpy() / np.maximum(n, 1.0)
 
 
-def run_pt_mcmc(gce, cfg):
-    """Run PT-MCMC (reference pt_mcmc_ce.py:601-1028): equilibration and production chunks, exchange at chunk
-    boundaries whose step is a multiple of the exchange interval, subsampled energies by temperature slot,
-    checkpoint / resume, energies.npz and mcmc_result.npz."""
-    import torch
-    os.makedirs(cfg.output_dir, exist_ok=True)
-    has_spin = bool(cfg.spin_orbits_pkl) and bool(cfg.spin_states)
-    energy_mode = getattr(cfg, "energy_mode", "full")
-    if energy_mode not in ("full", "delta"):
-        raise ValueError(f"cfg.energy_mode must be 'full' or 'delta'; got {energy_mode!r}")
-    if has_spin:
-        raise NotImplementedError("spin cluster expansions are not part of the gfx950 swap path")
-    precision = {"fp64": 0, "mixed": 1}[getattr(cfg, "precision", "fp64")]
+# ── the single-job driver ───────────────────────────────────────────────────────────────────────────────────────
+@dataclass
+class _Schedule:
+    """Chunk / sampling / exchange lengths of a run after reconciling them (reference pt_mcmc_ce.py:646-674)."""
+    chunk: int
+    sample_interval: int
+    exchange_interval: int
+    n_eq_chunks: int
+    n_prod_chunks: int
+
+    @classmethod
+    def of(cls, cfg, n_total):
+        chunk, sample, exch = int(cfg.chunk_size), int(cfg.sample_interval), int(cfg.exchange_interval)
+        if n_total > 1 and exch % chunk:
+            if exch < chunk:                  # exchanges more often than a chunk: the chunk shrinks to the interval
+                chunk = exch
+                if chunk % sample:
+                    sample = math.gcd(chunk, sample)
+                print(f"  exchange interval {exch} is shorter than chunk_size: chunks of {chunk} steps, samples "
+                      f"every {sample}", flush=True)
+            else:                             # rarer: the interval rounds to whole chunks
+                rounded = int(round(exch / chunk)) * chunk
+                print(f"  exchange interval {exch} is not a whole number of {chunk}-step chunks: using {rounded}",
+                      flush=True)
+                exch = rounded
+        n_prod = cfg.n_steps - cfg.n_equilib
+        for what, steps in (("n_equilib", cfg.n_equilib), ("n_steps - n_equilib", n_prod)):
+            if steps % chunk:
+                raise ValueError(f"{what} = {steps} is not a whole number of chunks of {chunk} steps")
+        if chunk % sample:
+            raise ValueError(f"sample_interval {sample} does not divide chunk_size {chunk}")
+        return cls(chunk, sample, exch, cfg.n_equilib // chunk, n_prod // chunk)
+
+
+def _ladder_of(cfg):
+    """temperatures, betas (float32 as the reference stores them), ladder id / position, n per ladder, n ladders."""
     n_ladders = max(int(cfg.n_ladders), 1)
-    n_per_ladder = int(cfg.n_replicas)
-    n_total = n_per_ladder * n_ladders
     T_grid = getattr(cfg, "T_grid", None)
     if T_grid is not None:
         if n_ladders != 1:
             raise ValueError("cfg.T_grid is one ladder; n_ladders must be 1")
-        T_grid = np.asarray(T_grid, dtype=np.float64).reshape(-1)
-        n_per_ladder = n_total = int(len(T_grid))
-        temps = T_grid.astype(np.float32)
-        betas = (1.0 / (_KB_EV_PER_K * T_grid)).astype(np.float32)
-        ladder_id = np.zeros(n_total, np.int32); ladder_pos = np.arange(n_total, dtype=np.int32)
-    elif n_ladders == 1:
-        temps, betas = make_temperature_ladder(cfg.T_min, cfg.T_max, n_per_ladder)
-        ladder_id = np.zeros(n_total, np.int32); ladder_pos = np.arange(n_total, dtype=np.int32)
-    else:
-        temps, betas, ladder_id, ladder_pos = _build_interleaved_ladders(cfg.T_min, cfg.T_max, n_per_ladder, n_ladders)
+        T = np.asarray(T_grid, dtype=np.float64).reshape(-1)
+        n = len(T)
+        return (T.astype(np.float32), (1.0 / (_KB_EV_PER_K * T)).astype(np.float32), np.zeros(n, np.int32),
+                np.arange(n, dtype=np.int32), n, 1)
+    n_per = int(cfg.n_replicas)
+    if n_ladders == 1:
+        temps, betas = make_temperature_ladder(cfg.T_min, cfg.T_max, n_per)
+        return temps, betas, np.zeros(n_per, np.int32), np.arange(n_per, dtype=np.int32), n_per, 1
+    temps, betas, lid, lpos = _build_interleaved_ladders(cfg.T_min, cfg.T_max, n_per, n_ladders)
+    return temps, betas, lid, lpos, n_per, n_ladders
 
-    chunk_size = int(cfg.chunk_size)
-    sample_interval = int(cfg.sample_interval)
-    exch_interval = int(cfg.exchange_interval)
-    if n_total > 1 and exch_interval % chunk_size != 0:
-        if exch_interval < chunk_size:
-            new_chunk = exch_interval
-            if new_chunk % sample_interval != 0:
-                sample_interval = math.gcd(new_chunk, sample_interval)
-            print(f"  chunk_size {chunk_size} -> {new_chunk} (= exchange interval) so exchanges fall on chunk "
-                  f"boundaries; sample_interval {sample_interval}", flush=True)
-            chunk_size = new_chunk
-        else:
-            new_ex = int(round(exch_interval / chunk_size)) * chunk_size
-            print(f"  exchange_interval {exch_interval} -> {new_ex} (whole chunks of {chunk_size})", flush=True)
-            exch_interval = new_ex
-    n_production = cfg.n_steps - cfg.n_equilib
-    if cfg.n_equilib % chunk_size != 0:
-        raise ValueError(f"n_equilib ({cfg.n_equilib}) must be a multiple of chunk_size ({chunk_size})")
-    if n_production % chunk_size != 0:
-        raise ValueError(f"(n_steps − n_equilib) = {n_production} must be a multiple of chunk_size ({chunk_size})")
-    if chunk_size % sample_interval != 0:
-        raise ValueError(f"chunk_size ({chunk_size}) must be a multiple of sample_interval ({sample_interval})")
 
-    print(f"PT-MCMC: {n_total} replicas, T=[{float(np.min(temps)):.0f}, {float(np.max(temps)):.0f}] K"
-          + (" (explicit list)" if T_grid is not None else "") + (f", {n_ladders} ladders" if n_ladders > 1 else ""))
-    print(f"  n_steps={cfg.n_steps:,}, n_equilib={cfg.n_equilib:,}, chunk_size={chunk_size:,}, exchange_interval="
-          f"{exch_interval}, sample_interval={sample_interval}, energy_mode={energy_mode}, precision="
-          f"{cfg.precision}", flush=True)
+def _checkpoint_want(cfg, sch, n_total, n_atoms):
+    return dict(n_steps=int(cfg.n_steps), chunk=sch.chunk, n_total=int(n_total), n_atoms=int(n_atoms),
+                sample_interval=sch.sample_interval, seed=int(cfg.seed), run_id=str(getattr(cfg, "run_id", "") or ""))
+
+
+def _save_checkpoint(path, c_done, si, samples, st, best, want, temps):
+    """the reference's checkpoint keys (pt_mcmc_ce.py:902-911: chunk, si, energy_samples, key, sigma, rep_of_slot,
+    beta_r, cfg_*, spins, E, sigma_min, E_min, swap_cum, flip_cum, ex_cum, step) plus this build's device state:
+    per-replica step counters, accepted counts, slot moments and the exchange counter."""
+    tmp = path + ".tmp.npz"
+    np.savez(tmp, chunk=np.int64(c_done), si=np.int64(si), energy_samples=samples[:si],
+             key=np.array([want["seed"]], np.int64), sigma=st.occ.cpu().numpy(),
+             rep_of_slot=st.rep_of_slot.cpu().numpy(), beta_r=st.beta_rep.cpu().numpy().astype(np.float32),
+             cfg_temps=np.asarray(temps), spins=np.zeros(0), E=st.energy.cpu().numpy(), sigma_min=best[1],
+             E_min=np.float64(best[0]), swap_cum=np.int64(st.n_accept.sum().item()), flip_cum=np.int64(0),
+             ex_cum=np.int64(st.n_swaps.item()), step=np.int32(int(st.step[0].item())),
+             step_rep=st.step.cpu().numpy(), n_accept=st.n_accept.cpu().numpy(),
+             acc_n=st.acc_n.cpu().numpy(), acc_mean=st.acc_mean.cpu().numpy(), acc_m2=st.acc_m2.cpu().numpy(),
+             e_ref=st.e_ref.cpu().numpy(), ex_count=st.ex_count.cpu().numpy(),
+             **{"cfg_" + k: v for k, v in want.items()})
+    os.replace(tmp, path)
+
+
+def _restore_checkpoint(path, st, want, temps):
+    """load a checkpoint into the device state; refuse one written by a run with other settings."""
+    import torch
+    ck = np.load(path, allow_pickle=False)
+    have = {k: (str(ck["cfg_" + k]) if isinstance(v, str) else int(ck["cfg_" + k])) for k, v in want.items()
+            if "cfg_" + k in ck.files}
+    if "run_id" not in have:                 # written without a run fingerprint: nothing to compare it with
+        have["run_id"] = want["run_id"]
+        print(f"  note: {path} carries no run_id; resuming without checking the model and start structures",
+              flush=True)
+    diff = sorted(k for k in want if have.get(k) != want[k])
+    if not np.allclose(np.asarray(ck["cfg_temps"], np.float64), np.asarray(temps, np.float64), rtol=0, atol=1e-6):
+        diff.append("temperatures")
+    if diff:
+        raise RuntimeError(f"{path} was written by another run (it differs in {', '.join(diff)}); remove the file "
+                           f"to start this run from the beginning")
+
+    def put(t, a):
+        t.copy_(torch.as_tensor(np.ascontiguousarray(a)).to(device=t.device, dtype=t.dtype))
+    put(st.occ, ck["sigma"]); put(st.energy, ck["E"]); put(st.rep_of_slot, ck["rep_of_slot"])
+    slot = np.empty(st.R, np.int32); slot[ck["rep_of_slot"]] = np.arange(st.R)
+    put(st.slot_of_rep, slot); put(st.beta_rep, ck["beta_r"].astype(np.float64))
+    put(st.step, ck["step_rep"]); put(st.n_accept, ck["n_accept"]); st.n_swaps.fill_(int(ck["ex_cum"]))
+    put(st.acc_n, ck["acc_n"]); put(st.acc_mean, ck["acc_mean"]); put(st.acc_m2, ck["acc_m2"])
+    put(st.e_ref, ck["e_ref"]); put(st.ex_count, ck["ex_count"])
+    st.e_min.copy_(st.energy); st.s1.zero_(); st.s2.zero_()
+    return int(ck["chunk"]), int(ck["si"]), ck["energy_samples"], (float(ck["E_min"]), ck["sigma_min"])
+
+
+def run_pt_mcmc(gce, cfg):
+    """Run PT-MCMC (reference pt_mcmc_ce.py:601-1028): equilibration and production chunks, an exchange after every
+    chunk whose last step is a multiple of the exchange interval, subsampled energies by temperature slot, the
+    lowest-energy configuration of production, checkpoint / resume, energies.npz and mcmc_result.npz."""
+    os.makedirs(cfg.output_dir, exist_ok=True)
+    energy_mode = getattr(cfg, "energy_mode", "full")
+    if energy_mode not in ("full", "delta"):
+        raise ValueError(f"cfg.energy_mode must be 'full' or 'delta'; got {energy_mode!r}")
+    if bool(cfg.spin_orbits_pkl) and bool(cfg.spin_states):
+        raise NotImplementedError("spin cluster expansions are not part of the gfx950 swap path")
+    precision = {"fp64": 0, "mixed": 1}[getattr(cfg, "precision", "fp64")]
+    tile = int(getattr(cfg, "tile", 0))
+    temps, betas, ladder_id, ladder_pos, n_per_ladder, n_ladders = _ladder_of(cfg)
+    n_total = len(temps)
+    sch = _Schedule.of(cfg, n_total)
     sl_species = [sl['chemical_symbols'] for sl in gce.sublattices]
-    print(f"  sublattices: {sl_species}")
     if cfg.swap_pairs:
-        sl_sets = [set(sp) for sp in sl_species]
-        bad = [(a, b) for (a, b) in cfg.swap_pairs if not any({a, b} <= s for s in sl_sets)]
+        bad = [tuple(p) for p in cfg.swap_pairs if not any(set(p) <= set(sp) for sp in sl_species)]
         if bad:
-            raise ValueError(f"swap_pairs {bad} do not lie within any single sublattice (sublattices: {sl_species}).")
+            raise ValueError(f"swap_pairs {bad} mix species of different sublattices {sl_species}")
     try:
         faulthandler.register(signal.SIGUSR1, all_threads=True)
     except (AttributeError, ValueError, RuntimeError):
         pass
+    print(f"PT-MCMC on the GPU: {n_total} temperatures {float(np.min(temps)):.0f}-{float(np.max(temps)):.0f} K"
+          + (f" in {n_ladders} interleaved ladders" if n_ladders > 1 else "")
+          + (" (T_grid)" if getattr(cfg, "T_grid", None) is not None else "")
+          + f", {gce.n_atoms} sites, sublattices {sl_species}")
+    print(f"  {cfg.n_steps:,} steps = {sch.n_eq_chunks} equilibration + {sch.n_prod_chunks} production chunks of "
+          f"{sch.chunk:,}; exchange every {sch.exchange_interval}, sample every {sch.sample_interval}; "
+          f"energy mode {energy_mode}, precision {cfg.precision}", flush=True)
 
     if cfg.initial_sigma is not None:
-        init_sigma = np.asarray(cfg.initial_sigma)
-        if init_sigma.shape not in ((gce.n_atoms,), (n_total, gce.n_atoms)):
-            raise ValueError(f"initial_sigma shape {init_sigma.shape} must be (n_atoms,)=({gce.n_atoms},) "
-                             f"or (n_replicas, n_atoms)=({n_total}, {gce.n_atoms})")
+        sig0 = np.asarray(cfg.initial_sigma)
+        if sig0.shape not in ((gce.n_atoms,), (n_total, gce.n_atoms)):
+            raise ValueError(f"initial_sigma has shape {sig0.shape}; expected ({gce.n_atoms},) or "
+                             f"({n_total}, {gce.n_atoms})")
     else:
-        init_sigma = make_initial_state(cfg.seed, gce, cfg.initial_composition, n_total)
-    samples_per_chunk = chunk_size // sample_interval
-    st = PTJobState(gce, init_sigma, betas.astype(np.float64), ladder_id if n_ladders > 1 else None,
-                    seed=cfg.seed, sample_interval=sample_interval, samples_per_chunk=samples_per_chunk)
-    n_chunks_eq = cfg.n_equilib // chunk_size
-    n_chunks_prod = n_production // chunk_size
-    n_chunks_tot = n_chunks_eq + n_chunks_prod
-    print(f"\nRunning {cfg.n_steps:,} steps  ({n_chunks_eq} equilib + {n_chunks_prod} production chunks)")
-    t0 = time.time()
-    progress_every = max(1, n_chunks_tot // 50)
-    energy_samples = np.zeros((n_chunks_prod * samples_per_chunk, n_total), dtype=np.float64)
-    si = 0; c0 = 0; n_ex_calls = 0
-    E_min = float(st.energy.min()); sigma_min = st.occ[int(st.energy.argmin())].cpu().numpy()
+        sig0 = make_initial_state(cfg.seed, gce, cfg.initial_composition, n_total)
+    per_chunk = sch.chunk // sch.sample_interval
+    st = PTJobState(gce, sig0, betas.astype(np.float64), ladder_id if n_ladders > 1 else None, seed=cfg.seed,
+                    sample_interval=sch.sample_interval, samples_per_chunk=per_chunk)
+    samples = np.zeros((sch.n_prod_chunks * per_chunk, n_total), dtype=np.float64)
+    si = 0
+    c_prod0 = 0
+    best = (float(st.energy.min()), st.occ[int(st.energy.argmin())].cpu().numpy())
     ck_every = int(getattr(cfg, "checkpoint_every", 0) or 0)
     ck_path = os.path.join(cfg.output_dir, "checkpoint.npz")
+    want = _checkpoint_want(cfg, sch, n_total, gce.n_atoms)
     if ck_every and os.path.exists(ck_path):
-        ck = np.load(ck_path, allow_pickle=False)
-        want = dict(n_steps=cfg.n_steps, chunk=chunk_size, n_total=n_total, n_atoms=gce.n_atoms,
-                    sample_interval=sample_interval, seed=cfg.seed)
-        have = {k: int(ck["cfg_" + k]) for k in want}
-        if "cfg_run_id" in ck.files:
-            want["run_id"] = str(getattr(cfg, "run_id", "") or ""); have["run_id"] = str(ck["cfg_run_id"])
-        if not np.allclose(np.asarray(ck["cfg_temps"]), np.asarray(temps), rtol=0, atol=1e-6):
-            have["temps"] = "differ"; want["temps"] = "same"
-        if have != want:
-            raise RuntimeError(f"checkpoint {ck_path} belongs to a different run (checkpoint {have} vs this run "
-                               f"{want}); delete it to start fresh")
-        c0 = int(ck["chunk"]); si = int(ck["si"]); energy_samples[:si] = ck["energy_samples"]
-        dev = gce.device
-        st.occ.copy_(torch.from_numpy(np.ascontiguousarray(ck["sigma"], np.int8)).to(dev))
-        st.energy.copy_(torch.from_numpy(ck["E"]).to(dev))
-        st.rep_of_slot.copy_(torch.from_numpy(ck["rep_of_slot"].astype(np.int32)).to(dev))
-        sor = np.empty(n_total, np.int32); sor[ck["rep_of_slot"]] = np.arange(n_total)
-        st.slot_of_rep.copy_(torch.from_numpy(sor).to(dev))
-        st.beta_rep.copy_(torch.from_numpy(ck["beta_r"].astype(np.float64)).to(dev))
-        st.step.copy_(torch.from_numpy(ck["step_rep"].astype(np.int64)).to(dev))
-        st.n_accept.copy_(torch.from_numpy(ck["n_accept"].astype(np.int64)).to(dev))
-        st.n_swaps.fill_(int(ck["ex_cum"]))
-        st.e_ref.copy_(st.energy); st.e_min.copy_(st.energy)
-        sigma_min = ck["sigma_min"]; E_min = float(ck["E_min"])
-        st.acc_n[:] = ck["acc_n"]; st.acc_mean[:] = ck["acc_mean"]; st.acc_m2[:] = ck["acc_m2"]
-        n_ex_calls = int(ck["n_ex_calls"])
-        print(f"  resumed from {ck_path}: production chunk {c0}/{n_chunks_prod}, {si} samples", flush=True)
-    chunks_this_run = (n_chunks_eq if c0 == 0 else 0) + (n_chunks_prod - c0)
+        c_prod0, si, done, best = _restore_checkpoint(ck_path, st, want, temps)
+        samples[:si] = done
+        print(f"  continuing from {ck_path}: {c_prod0} of {sch.n_prod_chunks} production chunks done", flush=True)
     ctx = gce.ctx
-    t_kernel = 0.0
+    n_chunks_all = sch.n_eq_chunks + sch.n_prod_chunks
+    todo = (0 if c_prod0 else sch.n_eq_chunks) + sch.n_prod_chunks - c_prod0
+    report_every = max(1, n_chunks_all // 50)
+    t0 = time.time()
+    t_launch = 0.0
+    n_run = 0
 
-    def one_chunk(accumulate):
-        nonlocal n_ex_calls, t_kernel
-        slot_before = st.slot_of_rep.cpu().numpy() if accumulate else None
+    def chunk(accumulate):
+        """one chunk of every replica, then the exchange if the global step count reached an exchange point."""
+        nonlocal t_launch
         tk = time.time()
-        ctx.pt_run_chunk([st.job(accumulate=accumulate)], chunk_size, precision, int(getattr(cfg, "tile", 0)))
-        step_now = int(st.step[0].item())          # synchronises
-        t_kernel += time.time() - tk
-        if accumulate:
-            st.merge_accumulators(chunk_size, slot_before)
-        if n_total > 1 and step_now % exch_interval == 0:
-            ctx.pt_exchange([st.exchange_job(step_now // exch_interval, reorder_last=True)])
-            n_ex_calls += 1
+        ctx.pt_run_chunk([st.job(accumulate=accumulate)], sch.chunk, precision, tile)
+        step_now = int(st.step[0].item())                       # waits for the chunk
+        t_launch += time.time() - tk
+        if n_total > 1 and step_now % sch.exchange_interval == 0:
+            ctx.pt_exchange([st.exchange_job(None, reorder_last=True)])
         return step_now
 
-    if c0 == 0:
-        for c in range(n_chunks_eq):
-            one_chunk(False)
-            if (c + 1) % progress_every == 0 or c + 1 == n_chunks_eq:
-                el = time.time() - t0; eta = el / (c + 1) * (chunks_this_run - (c + 1))
-                print(f"  [equil  {c+1:>5d}/{n_chunks_tot}]  E_min(curr)={float(st.energy.min()):>10.2f}  "
-                      f"elapsed={el:>7.1f}s  ETA={eta:>7.1f}s", flush=True)
-        idx0 = int(st.energy.argmin()); E_min = float(st.energy[idx0]); sigma_min = st.occ[idx0].cpu().numpy()
-        st.e_min.copy_(st.energy); st.e_ref.copy_(st.energy)
-        if n_chunks_eq > 0:
-            print("  → Equilibration complete; reset E_min for production.")
-    for c in range(c0, n_chunks_prod):
-        one_chunk(True)
-        E_sub = st.traj.cpu().numpy()
-        energy_samples[si:si + samples_per_chunk] = E_sub; si += samples_per_chunk
-        emin_r = st.e_min.cpu().numpy()
-        if emin_r.min() < E_min:                          # chunk-granular sigma_min, as pt_mcmc_ce.py:811-815
-            E_min = float(emin_r.min())
-            sigma_min = st.occ[int(st.energy.argmin())].cpu().numpy()
-        if ck_every and (c + 1) % ck_every == 0 and c + 1 < n_chunks_prod:
-            np.savez(ck_path + ".tmp.npz", chunk=c + 1, si=si, energy_samples=energy_samples[:si],
-                     key=np.array([cfg.seed], np.int64), sigma=st.occ.cpu().numpy(),
-                     rep_of_slot=st.rep_of_slot.cpu().numpy(), beta_r=st.beta_rep.cpu().numpy().astype(np.float32),
-                     cfg_n_steps=cfg.n_steps, cfg_chunk=chunk_size, cfg_n_total=n_total, cfg_n_atoms=gce.n_atoms,
-                     cfg_sample_interval=sample_interval, cfg_seed=cfg.seed, cfg_temps=np.asarray(temps),
-                     cfg_run_id=str(getattr(cfg, "run_id", "") or ""), spins=np.zeros(0),
-                     E=st.energy.cpu().numpy(), sigma_min=sigma_min, E_min=np.float64(E_min),
-                     swap_cum=np.int64(st.n_accept.sum().item()), flip_cum=np.int64(0),
-                     ex_cum=np.int64(st.n_swaps.item()), step=np.int32(int(st.step[0].item())),
-                     step_rep=st.step.cpu().numpy(), n_accept=st.n_accept.cpu().numpy(), acc_n=st.acc_n,
-                     acc_mean=st.acc_mean, acc_m2=st.acc_m2, n_ex_calls=np.int64(n_ex_calls))
-            os.replace(ck_path + ".tmp.npz", ck_path)
-        global_chunk = n_chunks_eq + c + 1
-        if global_chunk % progress_every == 0 or c + 1 == n_chunks_prod:
-            el = time.time() - t0
-            done = (n_chunks_eq if c0 == 0 else 0) + (c + 1 - c0)
-            eta = el / done * (chunks_this_run - done) if done else 0.0
-            print(f"  [prod   {global_chunk:>5d}/{n_chunks_tot}]  E_min(run)={E_min:>10.4f}  elapsed={el:>7.1f}s  "
-                  f"ETA={eta:>7.1f}s", flush=True)
-    wall_time = time.time() - t0
-    n_swap_attempts = cfg.n_steps * n_total
-    pairs_per_step = (n_per_ladder // 2) * n_ladders
-    n_ex_attempts = (cfg.n_steps // exch_interval) * pairs_per_step
-    swap_accept_rate = float(st.n_accept.sum().item()) / max(1, n_swap_attempts)
-    exchange_accept_rate = float(st.n_swaps.item()) / max(1, n_ex_attempts)
-    attempts_per_s = n_total * chunk_size * chunks_this_run / max(t_kernel, 1e-12)
-    print(f"\nDone in {wall_time:.1f}s  ({attempts_per_s:,.0f} swap attempts/s in the chunk kernel)")
-    print(f"  swap acceptance     : {swap_accept_rate:.4f}")
-    print(f"  exchange acceptance : {exchange_accept_rate:.4f}")
-    print(f"  E_min (production)  : {E_min:.4f}", flush=True)
-    temps_np = np.asarray(temps)
+    def report(phase, k, figure):
+        nonlocal n_run
+        n_run += 1
+        if k % report_every and k != n_chunks_all and not (phase == "equilibration" and k == sch.n_eq_chunks):
+            return
+        el = time.time() - t0
+        print(f"  {phase:<13s} chunk {k:>5d}/{n_chunks_all}  {figure}  {el:7.1f} s elapsed, "
+              f"{el / n_run * (todo - n_run):7.1f} s to go", flush=True)
+
+    if c_prod0 == 0:
+        for c in range(sch.n_eq_chunks):
+            chunk(False)
+            report("equilibration", c + 1, f"lowest E now {float(st.energy.min()):12.4f}")
+        if sch.n_eq_chunks:
+            i0 = int(st.energy.argmin())
+            best = (float(st.energy[i0]), st.occ[i0].cpu().numpy())
+            print("  equilibration done; the lowest energy is tracked from production on", flush=True)
+        st.e_min.copy_(st.energy)
+        st.reset_moments()
+    for c in range(c_prod0, sch.n_prod_chunks):
+        chunk(True)
+        samples[si:si + per_chunk] = st.traj.cpu().numpy()
+        si += per_chunk
+        lows = st.e_min.cpu().numpy()
+        if lows.min() < best[0]:                            # chunk granularity, as pt_mcmc_ce.py:811-815
+            best = (float(lows.min()), st.occ[int(st.energy.argmin())].cpu().numpy())
+        if ck_every and (c + 1) % ck_every == 0 and c + 1 < sch.n_prod_chunks:
+            _save_checkpoint(ck_path, c + 1, si, samples, st, best, want, temps)
+        report("production", sch.n_eq_chunks + c + 1, f"lowest E {best[0]:12.4f}")
+    wall = time.time() - t0
+    attempts = cfg.n_steps * n_total
+    ex_pairs = (cfg.n_steps // sch.exchange_interval) * (n_per_ladder // 2) * n_ladders
+    swap_rate = float(st.n_accept.sum().item()) / max(1, attempts)
+    ex_rate = float(st.n_swaps.item()) / max(1, ex_pairs)
+    rate = n_total * sch.chunk * todo / max(t_launch, 1e-12)
+    E_min = best[0]
+    print(f"\nfinished in {wall:.1f} s, {rate:,.0f} swap attempts/s in the chunk launches; acceptance: swaps "
+          f"{swap_rate:.4f}, exchanges {ex_rate:.4f}; lowest production energy {E_min:.4f}", flush=True)
     E_mean, E_var = st.moments()
-    np.savez(os.path.join(cfg.output_dir, 'energies.npz'), energies=energy_samples[:si], temperatures=temps_np,
-             ladder_id=np.asarray(ladder_id), n_atoms=np.int32(gce.n_atoms), swap_accept_rate=swap_accept_rate,
-             exchange_accept_rate=exchange_accept_rate, flip_accept_rate=0.0, E_min=E_min, wall_time=wall_time,
-             E_mean=E_mean, E_var=E_var, n_accum=st.acc_n)
-    print(f"  wrote {os.path.join(cfg.output_dir, 'energies.npz')} ({si} samples x {n_total} T)", flush=True)
-    ros = st.rep_of_slot.cpu().numpy()
-    sigma_final = st.occ.cpu().numpy()[ros].astype(np.int32)
+    temps_np = np.asarray(temps)
+    out = lambda name: os.path.join(cfg.output_dir, name)          # noqa: E731
+    np.savez(out("energies.npz"), energies=samples[:si], temperatures=temps_np, ladder_id=np.asarray(ladder_id),
+             n_atoms=np.int32(gce.n_atoms), swap_accept_rate=swap_rate, exchange_accept_rate=ex_rate,
+             flip_accept_rate=0.0, E_min=E_min, wall_time=wall, E_mean=E_mean, E_var=E_var,
+             n_accum=st.acc_n.cpu().numpy())
+    print(f"  {out('energies.npz')}: {si} samples x {n_total} temperatures", flush=True)
     ip = np.asarray(gce.inv_perm)
-    sigma_final = np.ascontiguousarray(sigma_final)[:, ip]
-    sigma_min_o = np.asarray(sigma_min).astype(np.int32)[ip]
-    result = MCMCResult(energies=energy_samples[:si], temperatures=temps_np, swap_accept_rate=swap_accept_rate,
-                        flip_accept_rate=0.0, exchange_accept_rate=exchange_accept_rate, sigma_final=sigma_final,
-                        sigma_min=sigma_min_o, E_min=E_min, spins_final=None, wall_time=wall_time,
-                        E_mean=E_mean, E_var=E_var, attempts_per_s=attempts_per_s)
-    np.savez(os.path.join(cfg.output_dir, 'mcmc_result.npz'), energies=result.energies,
-             temperatures=result.temperatures, ladder_id=np.asarray(ladder_id), ladder_pos=np.asarray(ladder_pos),
-             n_ladders=np.int32(n_ladders), sigma_final=result.sigma_final, sigma_min=result.sigma_min,
-             E_min=result.E_min, spins_final=np.array([]), swap_accept_rate=result.swap_accept_rate,
-             flip_accept_rate=0.0, exchange_accept_rate=result.exchange_accept_rate, wall_time=result.wall_time)
-    print(f"  wrote {os.path.join(cfg.output_dir, 'mcmc_result.npz')}", flush=True)
+    by_slot = st.occ.cpu().numpy()[st.rep_of_slot.cpu().numpy()].astype(np.int32)
+    result = MCMCResult(energies=samples[:si], temperatures=temps_np, swap_accept_rate=swap_rate,
+                        flip_accept_rate=0.0, exchange_accept_rate=ex_rate,
+                        sigma_final=np.ascontiguousarray(by_slot[:, ip]),
+                        sigma_min=np.asarray(best[1]).astype(np.int32)[ip], E_min=E_min, spins_final=None,
+                        wall_time=wall, E_mean=E_mean, E_var=E_var, attempts_per_s=rate)
+    np.savez(out("mcmc_result.npz"), energies=result.energies, temperatures=result.temperatures,
+             ladder_id=np.asarray(ladder_id), ladder_pos=np.asarray(ladder_pos), n_ladders=np.int32(n_ladders),
+             sigma_final=result.sigma_final, sigma_min=result.sigma_min, E_min=result.E_min, spins_final=np.array([]),
+             swap_accept_rate=result.swap_accept_rate, flip_accept_rate=0.0,
+             exchange_accept_rate=result.exchange_accept_rate, wall_time=result.wall_time)
+    print(f"  {out('mcmc_result.npz')}: final and lowest-energy configurations", flush=True)
     if ck_every and os.path.exists(ck_path):
         os.remove(ck_path)
     return result

@@ -177,8 +177,14 @@ struct nce_ctx {
         int next = 0;
     };
     std::vector<StageRing *> rings;
-    double *d_partial = nullptr;
-    size_t partial_cap = 0;
+    // Per-block partial sums of the full-energy kernels, one buffer per stream (like the staging rings: a buffer
+    // shared by streams could be rewritten on one stream while a launch on another still reads it).
+    struct Partial {
+        hipStream_t stream = nullptr;
+        double *p = nullptr;
+        size_t cap = 0;
+    };
+    std::vector<Partial> partials;
     std::vector<NNModelDev> nn_models;
     void *d_nn_models = nullptr;
     int cap_nn_models = 0;
@@ -1261,11 +1267,13 @@ __global__ __launch_bounds__(EXCH_BS) void pt_exchange_kernel(const nce_pt_excha
     const nce_pt_exchange_job j = jobs[blockIdx.x];
     const int R = j.n_replicas;
     const uint32_t k0 = (uint32_t)j.seed, k1 = (uint32_t)(j.seed >> 32);
+    // exchange counter from the device when the job gives one (captured cycles), else from the job table
+    const uint64_t ex_index = j.ex_count ? (uint64_t)(j.ex_count[0] + 1) : j.ex_index;
     int parity;
     if (j.inj_parity >= 0) {
         parity = j.inj_parity & 1;
     } else {
-        uint32_t z[4] = {(uint32_t)j.ex_index, (uint32_t)(j.ex_index >> 32), 0xffffffffu, 2u};
+        uint32_t z[4] = {(uint32_t)ex_index, (uint32_t)(ex_index >> 32), 0xffffffffu, 2u};
         philox4x32_10(z, k0, k1);
         parity = (int)(z[0] & 1u);
     }
@@ -1280,7 +1288,7 @@ __global__ __launch_bounds__(EXCH_BS) void pt_exchange_kernel(const nce_pt_excha
         if (j.inj_u) {
             u = j.inj_u[p];
         } else {
-            uint32_t w[4] = {(uint32_t)j.ex_index, (uint32_t)(j.ex_index >> 32), (uint32_t)p, 3u};
+            uint32_t w[4] = {(uint32_t)ex_index, (uint32_t)(ex_index >> 32), (uint32_t)p, 3u};
             philox4x32_10(w, k0, k1);
             u = ((double)((((uint64_t)w[1] << 32) | w[0]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
         }
@@ -1300,7 +1308,39 @@ __global__ __launch_bounds__(EXCH_BS) void pt_exchange_kernel(const nce_pt_excha
         j.slot_of_rep[rep] = t;
         if (j.traj_last) j.traj_last[t] = j.energy[rep];
     }
-    if (threadIdx.x == 0) j.n_swaps[0] += s_ns;
+    if (threadIdx.x == 0) {
+        j.n_swaps[0] += s_ns;
+        if (j.ex_count) j.ex_count[0] = (int64_t)ex_index;     // every thread read it before the barrier above
+    }
+}
+
+// Moments of a chunk folded into the per-slot accumulators (Chan, Golub, LeVeque: pairwise update of count, mean and
+// sum of squared deviations), one thread per (job, replica).  slot_of_rep is a permutation within a job, so every
+// slot is written by exactly one thread.  Runs after the chunk and before the exchange that may move the slots.
+constexpr int MERGE_BS = 256;
+
+__global__ __launch_bounds__(MERGE_BS) void pt_merge_kernel(const nce_pt_job *__restrict__ jobs,
+                                                            const int2 *__restrict__ items, int n_items,
+                                                            int64_t n_steps) {
+    const int i = blockIdx.x * MERGE_BS + threadIdx.x;
+    if (i >= n_items) return;
+    const int2 it = items[i];
+    const nce_pt_job j = jobs[it.x];
+    if (!j.acc_n || !j.accumulate) return;
+    const int r = it.y;
+    const int t = j.slot_of_rep[r];
+    const double n = (double)n_steps;
+    const double s1 = j.s1[r], s2 = j.s2[r];
+    const double mean_c = __dadd_rn(j.e_ref[r], s1 / n);
+    const double m2_c = fmax(__dadd_rn(s2, -__dmul_rn(s1, s1) / n), 0.0);
+    const double na = j.acc_n[t], tot = __dadd_rn(na, n);
+    const double delta = __dadd_rn(mean_c, -j.acc_mean[t]);
+    j.acc_mean[t] = __dadd_rn(j.acc_mean[t], __dmul_rn(delta, n) / tot);
+    j.acc_m2[t] = __dadd_rn(j.acc_m2[t], __dadd_rn(m2_c, __dmul_rn(__dmul_rn(__dmul_rn(delta, delta), na), n) / tot));
+    j.acc_n[t] = tot;
+    j.s1[r] = 0.0;
+    j.s2[r] = 0.0;
+    const_cast<double *>(j.e_ref)[r] = j.energy[r];
 }
 
 // ───────────────────────────────────────────── host side ──────────────────────────────────────────
@@ -1328,7 +1368,8 @@ extern "C" void nce_ctx_destroy(nce_ctx *c) {
     (void)hipDeviceSynchronize();
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->d_models) (void)hipFree(c->d_models);
-    if (c->d_partial) (void)hipFree(c->d_partial);
+    for (nce_ctx::Partial &pb : c->partials)
+        if (pb.p) (void)hipFree(pb.p);
     if (c->d_nn_models) (void)hipFree(c->d_nn_models);
     for (nce_ctx::StageRing *ring : c->rings) {
         for (auto &st : ring->stage) {
@@ -1556,6 +1597,30 @@ static int check_model(nce_ctx *c, int32_t model) {
     return NCE_OK;
 }
 
+// the partial-sum buffer of stream s, grown to `need` doubles (the old buffer is freed after the stream drains)
+static int partial_for(nce_ctx *c, hipStream_t s, size_t need, double **out) {
+    nce_ctx::Partial *pb = nullptr;
+    for (nce_ctx::Partial &q : c->partials)
+        if (q.stream == s) pb = &q;
+    if (!pb) {
+        c->partials.push_back(nce_ctx::Partial());
+        pb = &c->partials.back();
+        pb->stream = s;
+    }
+    if (need > pb->cap) {
+        if (pb->p) {
+            HIP_TRY(hipStreamSynchronize(s));
+            HIP_TRY(hipFree(pb->p));
+            pb->p = nullptr;
+            pb->cap = 0;
+        }
+        HIP_TRY(hipMalloc(&pb->p, need * sizeof(double)));
+        pb->cap = need;
+    }
+    *out = pb->p;
+    return NCE_OK;
+}
+
 template <typename K>
 static int allow_lds(K kernel, size_t lds) {
     if (lds > 65536)
@@ -1575,22 +1640,15 @@ extern "C" int nce_ce_energy(nce_ctx *c, int32_t model, const int8_t *occ, int32
     if (nblk64 > (1 << 30)) return set_err(NCE_ERR_ARG, "cell too large");
     const int nblk = (int)nblk64;
     const size_t need = (size_t)R * nblk;
-    if (need > c->partial_cap) {
-        if (c->d_partial) {
-            HIP_TRY(hipDeviceSynchronize());
-            HIP_TRY(hipFree(c->d_partial));
-        }
-        HIP_TRY(hipMalloc(&c->d_partial, need * sizeof(double)));
-        c->partial_cap = need;
-    }
+    hipStream_t s = (hipStream_t)stream;
+    double *partial = nullptr;
+    if (int rc1 = partial_for(c, s, need, &partial)) return rc1;
     const size_t lds = c->lds_energy[model];
     if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "forward rows do not fit in LDS");
-    hipStream_t s = (hipStream_t)stream;
     if (int rc2 = allow_lds(ce_energy_kernel, lds)) return rc2;
-    hipLaunchKernelGGL(ce_energy_kernel, dim3(nblk, R), dim3(ENERGY_BS), lds, s, c->d_models, model, occ, N,
-                       c->d_partial);
+    hipLaunchKernelGGL(ce_energy_kernel, dim3(nblk, R), dim3(ENERGY_BS), lds, s, c->d_models, model, occ, N, partial);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(ce_energy_finish_kernel, dim3(R), dim3(64), 0, s, c->d_models, model, c->d_partial, nblk, E);
+    hipLaunchKernelGGL(ce_energy_finish_kernel, dim3(R), dim3(64), 0, s, c->d_models, model, partial, nblk, E);
     HIP_TRY(hipGetLastError());
     return NCE_OK;
 }
@@ -1654,37 +1712,35 @@ static int stage_upload(nce_ctx *c, const void *host, size_t bytes, hipStream_t 
     return NCE_OK;
 }
 
-template <typename Real, int W>
-static int launch_chunk(nce_ctx *c, const nce_pt_job *d_jobs, const int2 *d_items, int n_items, int64_t n_steps,
-                        size_t lds, hipStream_t s) {
-    if (int rc = allow_lds(ce_chunk_kernel<Real, W>, lds)) return rc;
-    hipLaunchKernelGGL((ce_chunk_kernel<Real, W>), dim3(n_items), dim3(64 * W), lds, s, c->d_models, d_jobs, d_items,
-                       n_steps);
-    HIP_TRY(hipGetLastError());
-    return NCE_OK;
-}
+// A chunk launch decided once: the job table and work items (one staged buffer), the kernel instance, its block
+// size and LDS bytes, and whether the moment merge follows.  Shared by nce_pt_run_chunk and the captured cycles.
+typedef void (*ChunkKernel)(const ModelDev *, const nce_pt_job *, const int2 *, int64_t);
+
+struct ChunkPlan {
+    std::vector<unsigned char> table;                 // [n_jobs] nce_pt_job (16-aligned) then [n_items] int2
+    size_t jb_bytes = 0;
+    int n_jobs = 0, n_items = 0, block = 0;
+    size_t lds = 0;
+    int64_t n_steps = 0;
+    bool inj = false, merge = false;
+    ChunkKernel kernel = nullptr;
+};
 
 template <typename Real, int W>
-static int launch_chunk_spec(nce_ctx *c, const nce_pt_job *d_jobs, const int2 *d_items, int n_items,
-                             int64_t n_steps, size_t lds, hipStream_t s, bool inj) {
-    auto k = inj ? ce_chunk_spec_kernel<Real, W, true> : ce_chunk_spec_kernel<Real, W, false>;
-    if (int rc = allow_lds(k, lds)) return rc;
-    hipLaunchKernelGGL(k, dim3(n_items), dim3(64 * W), lds, s, c->d_models, d_jobs, d_items, n_steps);
-    HIP_TRY(hipGetLastError());
-    return NCE_OK;
+static ChunkKernel general_kernel() { return ce_chunk_kernel<Real, W>; }
+template <typename Real, int W>
+static ChunkKernel spec_kernel(bool inj) {
+    return inj ? ce_chunk_spec_kernel<Real, W, true> : ce_chunk_spec_kernel<Real, W, false>;
 }
 
-static int wave_index(int W) { return W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : 3; }
-
-extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps,
-                                int32_t precision, int32_t tile, void *stream) {
+static int plan_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps, int32_t precision,
+                      int32_t tile, ChunkPlan &P) {
     if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
     if (n_steps < 0) return set_err(NCE_ERR_ARG, "negative n_steps");
     if (precision != NCE_PREC_FP64 && precision != NCE_PREC_MIXED) return set_err(NCE_ERR_ARG, "unknown precision");
-    if (n_steps == 0) return NCE_OK;
-    HIP_TRY(hipSetDevice(c->device));
     std::vector<int2> items;
     size_t lds_need = 0;
+    bool merge = false;
     for (int j = 0; j < n_jobs; ++j) {
         const nce_pt_job &jb = jobs[j];
         int rc = check_model(c, jb.model);
@@ -1697,8 +1753,12 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
         if (jb.traj && (jb.sample_interval < 1 || n_steps % jb.sample_interval != 0))
             return set_err(NCE_ERR_ARG, "n_steps must be a multiple of sample_interval");
         if (jb.accumulate && (!jb.e_ref || !jb.s1 || !jb.s2)) return set_err(NCE_ERR_ARG, "accumulators missing");
+        if ((jb.acc_n != nullptr) != (jb.acc_mean != nullptr) || (jb.acc_n != nullptr) != (jb.acc_m2 != nullptr))
+            return set_err(NCE_ERR_ARG, "slot moments acc_n / acc_mean / acc_m2 must be all set or all NULL");
+        if (jb.acc_n && !jb.accumulate) return set_err(NCE_ERR_ARG, "slot moments need accumulate = 1");
         if ((jb.inj_ia != nullptr) != (jb.inj_ib != nullptr) || (jb.inj_ia != nullptr) != (jb.inj_u != nullptr))
             return set_err(NCE_ERR_ARG, "injected streams must be all set or all NULL");
+        merge = merge || jb.acc_n != nullptr;
         for (int r = 0; r < jb.n_replicas; ++r) items.push_back(make_int2(j, r));
         lds_need = std::max(lds_need, precision == NCE_PREC_FP64 ? c->lds_chunk64[jb.model] : c->lds_chunk32[jb.model]);
     }
@@ -1720,18 +1780,14 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
         }
         while (tile_spec > 4 && 0.5 * tile_spec * (tile_spec - 1) * 4.0 * dens > 2.0) tile_spec /= 2;
     }
-    // job table and work items in one staged buffer
-    const size_t jb_bytes = align16(sizeof(nce_pt_job) * n_jobs);
-    std::vector<unsigned char> buf(jb_bytes + items.size() * sizeof(int2));
-    std::memcpy(buf.data(), jobs, sizeof(nce_pt_job) * n_jobs);
-    std::memcpy(buf.data() + jb_bytes, items.data(), items.size() * sizeof(int2));
-    hipStream_t s = (hipStream_t)stream;
-    void *dev = nullptr;
-    int rc = stage_upload(c, buf.data(), buf.size(), s, &dev);
-    if (rc) return rc;
-    const nce_pt_job *d_jobs = (const nce_pt_job *)dev;
-    const int2 *d_items = (const int2 *)((unsigned char *)dev + jb_bytes);
-    const int n_items = (int)items.size();
+    P.jb_bytes = align16(sizeof(nce_pt_job) * n_jobs);
+    P.table.assign(P.jb_bytes + items.size() * sizeof(int2), 0);
+    std::memcpy(P.table.data(), jobs, sizeof(nce_pt_job) * n_jobs);
+    std::memcpy(P.table.data() + P.jb_bytes, items.data(), items.size() * sizeof(int2));
+    P.n_jobs = n_jobs;
+    P.n_items = (int)items.size();
+    P.n_steps = n_steps;
+    P.merge = merge;
     // recompute the LDS bytes for the chosen wave count (reduction scratch depends on it)
     size_t lds = 0;
     for (int j = 0; j < n_jobs; ++j) {
@@ -1741,7 +1797,6 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
     }
     bool fast = true;
     const bool inj = jobs[0].inj_ia != nullptr;
-    size_t lds_fast = 0;
     int SW = tile_spec;
     auto spec_lds = [&](int waves) {
         size_t need = 0;
@@ -1759,37 +1814,56 @@ extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jo
         if ((jobs[j].inj_ia != nullptr) != inj) fast = false;          // one stream kind per launch
     }
     if (SW == 16 && spec_lds(16) > 160 * 1024) SW = 8;
-    lds_fast = spec_lds(SW);
+    const size_t lds_fast = spec_lds(SW);
     const char *fg = std::getenv("NCE_FORCE_GENERAL");               // tests: run the general kernel on any model
     if (fg && fg[0] == '1') fast = false;
+    P.inj = inj;
     if (fast && lds_fast <= 160 * 1024) {
-        if (precision == NCE_PREC_FP64) {
-            switch (SW) {
-                case 4: return launch_chunk_spec<double, 4>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
-                case 16: return launch_chunk_spec<double, 16>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
-                default: return launch_chunk_spec<double, 8>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
-            }
-        }
-        switch (SW) {
-            case 4: return launch_chunk_spec<float, 4>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
-            case 16: return launch_chunk_spec<float, 16>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
-            default: return launch_chunk_spec<float, 8>(c, d_jobs, d_items, n_items, n_steps, lds_fast, s, inj);
-        }
+        const bool d = precision == NCE_PREC_FP64;
+        P.kernel = SW == 4 ? (d ? spec_kernel<double, 4>(inj) : spec_kernel<float, 4>(inj))
+                 : SW == 16 ? (d ? spec_kernel<double, 16>(inj) : spec_kernel<float, 16>(inj))
+                            : (d ? spec_kernel<double, 8>(inj) : spec_kernel<float, 8>(inj));
+        P.block = 64 * SW;
+        P.lds = lds_fast;
+    } else {
+        const bool d = precision == NCE_PREC_FP64;
+        P.kernel = W == 1 ? (d ? general_kernel<double, 1>() : general_kernel<float, 1>())
+                 : W == 2 ? (d ? general_kernel<double, 2>() : general_kernel<float, 2>())
+                 : W == 4 ? (d ? general_kernel<double, 4>() : general_kernel<float, 4>())
+                          : (d ? general_kernel<double, 8>() : general_kernel<float, 8>());
+        P.block = 64 * (W > 8 ? 8 : W);
+        P.lds = lds;
     }
-    if (precision == NCE_PREC_FP64) {
-        switch (W) {
-            case 1: return launch_chunk<double, 1>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-            case 2: return launch_chunk<double, 2>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-            case 4: return launch_chunk<double, 4>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-            default: return launch_chunk<double, 8>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-        }
+    if (P.lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "template does not fit in LDS");
+    return allow_lds(P.kernel, P.lds);
+}
+
+// the chunk kernel of a plan on its device tables, then the moment merge when a job asks for it
+static int launch_plan(const ChunkPlan &P, const ModelDev *d_models, const void *d_table, hipStream_t s) {
+    const nce_pt_job *d_jobs = (const nce_pt_job *)d_table;
+    const int2 *d_items = (const int2 *)((const unsigned char *)d_table + P.jb_bytes);
+    int64_t n_steps = P.n_steps;
+    void *args[] = {(void *)&d_models, (void *)&d_jobs, (void *)&d_items, (void *)&n_steps};
+    HIP_TRY(hipLaunchKernel((const void *)P.kernel, dim3(P.n_items), dim3(P.block), args, P.lds, s));
+    if (P.merge) {
+        const int nb = (P.n_items + MERGE_BS - 1) / MERGE_BS;
+        hipLaunchKernelGGL(pt_merge_kernel, dim3(nb), dim3(MERGE_BS), 0, s, d_jobs, d_items, P.n_items, P.n_steps);
+        HIP_TRY(hipGetLastError());
     }
-    switch (W) {
-        case 1: return launch_chunk<float, 1>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-        case 2: return launch_chunk<float, 2>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-        case 4: return launch_chunk<float, 4>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-        default: return launch_chunk<float, 8>(c, d_jobs, d_items, n_items, n_steps, lds, s);
-    }
+    return NCE_OK;
+}
+
+extern "C" int nce_pt_run_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps,
+                                int32_t precision, int32_t tile, void *stream) {
+    if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
+    if (n_steps == 0) return NCE_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    ChunkPlan P;
+    if (int rc = plan_chunk(c, jobs, n_jobs, n_steps, precision, tile, P)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    void *dev = nullptr;
+    if (int rc = stage_upload(c, P.table.data(), P.table.size(), s, &dev)) return rc;
+    return launch_plan(P, c->d_models, dev, s);
 }
 
 extern "C" int nce_pt_exchange(nce_ctx *c, const nce_pt_exchange_job *jobs, int32_t n_jobs, void *stream) {
@@ -1807,6 +1881,99 @@ extern "C" int nce_pt_exchange(nce_ctx *c, const nce_pt_exchange_job *jobs, int3
     if (rc) return rc;
     hipLaunchKernelGGL(pt_exchange_kernel, dim3(n_jobs), dim3(EXCH_BS), 0, s, (const nce_pt_exchange_job *)dev);
     HIP_TRY(hipGetLastError());
+    return NCE_OK;
+}
+
+// ─────────────────────────────────── captured exchange cycles (HIP graphs) ───────────────────────────────────
+struct nce_pt_cycle {
+    nce_ctx *c = nullptr;
+    void *d_mem = nullptr;            // model descriptors (a snapshot: the context's array may be reallocated),
+                                      // job table + items, exchange table
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" void nce_pt_cycle_destroy(nce_pt_cycle *cy) {
+    if (!cy) return;
+    if (cy->c) (void)hipSetDevice(cy->c->device);
+    if (cy->exec) (void)hipGraphExecDestroy(cy->exec);
+    if (cy->graph) (void)hipGraphDestroy(cy->graph);
+    if (cy->d_mem) {
+        (void)hipDeviceSynchronize();                 // a replay still in flight reads the tables
+        (void)hipFree(cy->d_mem);
+    }
+    delete cy;
+}
+
+extern "C" int nce_pt_cycle_create(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jobs, int64_t n_steps,
+                                   int32_t chunks_per_cycle, int32_t precision, int32_t tile,
+                                   const nce_pt_exchange_job *ex_jobs, int32_t n_ex_jobs, nce_pt_cycle **out) {
+    if (!out) return set_err(NCE_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!c || !jobs || n_jobs < 1) return set_err(NCE_ERR_ARG, "NULL context / jobs");
+    if (n_steps < 1 || chunks_per_cycle < 1) return set_err(NCE_ERR_ARG, "a cycle needs n_steps >= 1 and chunks >= 1");
+    if (n_ex_jobs < 0 || (n_ex_jobs > 0 && !ex_jobs)) return set_err(NCE_ERR_ARG, "bad exchange table");
+    for (int j = 0; j < n_jobs; ++j)
+        if (jobs[j].inj_ia) return set_err(NCE_ERR_ARG, "injected streams cannot be replayed by a cycle");
+    for (int j = 0; j < n_ex_jobs; ++j) {
+        const nce_pt_exchange_job &e = ex_jobs[j];
+        if (e.n_replicas < 1 || !e.energy || !e.rep_of_slot || !e.slot_of_rep || !e.beta_slot || !e.beta_rep ||
+            !e.n_swaps)
+            return set_err(NCE_ERR_ARG, "exchange job has a NULL buffer");
+        if (!e.ex_count) return set_err(NCE_ERR_ARG, "a cycle's exchange jobs must give ex_count");
+        if (e.inj_u || e.inj_parity >= 0) return set_err(NCE_ERR_ARG, "injected exchanges cannot be replayed");
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    ChunkPlan P;
+    if (int rc = plan_chunk(c, jobs, n_jobs, n_steps, precision, tile, P)) return rc;
+    const size_t models_bytes = align16(c->models.size() * sizeof(ModelDev));
+    const size_t table_bytes = align16(P.table.size());
+    const size_t ex_bytes = sizeof(nce_pt_exchange_job) * (size_t)n_ex_jobs;
+    nce_pt_cycle *cy = new nce_pt_cycle();
+    cy->c = c;
+    auto fail = [&](int rc) {
+        nce_pt_cycle_destroy(cy);
+        return rc;
+    };
+    if (hipMalloc(&cy->d_mem, models_bytes + table_bytes + ex_bytes) != hipSuccess)
+        return fail(set_err(NCE_ERR_NOMEM, "cycle tables: hipMalloc failed"));
+    unsigned char *base = (unsigned char *)cy->d_mem;
+    std::vector<unsigned char> host(models_bytes + table_bytes + ex_bytes, 0);
+    std::memcpy(host.data(), c->models.data(), c->models.size() * sizeof(ModelDev));
+    std::memcpy(host.data() + models_bytes, P.table.data(), P.table.size());
+    if (n_ex_jobs) std::memcpy(host.data() + models_bytes + table_bytes, ex_jobs, ex_bytes);
+    if (hipMemcpy(cy->d_mem, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(NCE_ERR_HIP, "cycle tables: hipMemcpy failed"));
+    hipStream_t cap = nullptr;
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(NCE_ERR_HIP, "cycle: hipStreamCreateWithFlags failed"));
+    int rc = NCE_OK;
+    if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        rc = set_err(NCE_ERR_HIP, "cycle: hipStreamBeginCapture failed");
+    } else {
+        for (int k = 0; k < chunks_per_cycle && rc == NCE_OK; ++k)
+            rc = launch_plan(P, (const ModelDev *)base, base + models_bytes, cap);
+        if (rc == NCE_OK && n_ex_jobs) {
+            hipLaunchKernelGGL(pt_exchange_kernel, dim3(n_ex_jobs), dim3(EXCH_BS), 0, cap,
+                               (const nce_pt_exchange_job *)(base + models_bytes + table_bytes));
+            if (hipGetLastError() != hipSuccess) rc = set_err(NCE_ERR_HIP, "cycle: exchange launch failed");
+        }
+        const hipError_t e = hipStreamEndCapture(cap, &cy->graph);      // always end the capture
+        if (rc == NCE_OK && e != hipSuccess) rc = set_err(NCE_ERR_HIP, "cycle: hipStreamEndCapture failed");
+    }
+    (void)hipStreamDestroy(cap);
+    if (rc != NCE_OK) return fail(rc);
+    if (hipGraphInstantiate(&cy->exec, cy->graph, nullptr, nullptr, 0) != hipSuccess)
+        return fail(set_err(NCE_ERR_HIP, "cycle: hipGraphInstantiate failed"));
+    *out = cy;
+    return NCE_OK;
+}
+
+extern "C" int nce_pt_cycle_launch(nce_pt_cycle *cy, int32_t n_cycles, void *stream) {
+    if (!cy || !cy->exec) return set_err(NCE_ERR_ARG, "NULL cycle");
+    if (n_cycles < 0) return set_err(NCE_ERR_ARG, "negative cycle count");
+    HIP_TRY(hipSetDevice(cy->c->device));
+    for (int k = 0; k < n_cycles; ++k) HIP_TRY(hipGraphLaunch(cy->exec, (hipStream_t)stream));
     return NCE_OK;
 }
 

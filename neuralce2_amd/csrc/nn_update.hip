@@ -1511,17 +1511,11 @@ static int nn_full_cache_impl(nce_ctx *c, int32_t model, const int8_t *occ, int3
     }
     const int nbe = (int)((N + 255) / 256);
     const size_t need = (size_t)R * nbe;
-    if (need > c->partial_cap) {
-        if (c->d_partial) {
-            HIP_TRY(hipDeviceSynchronize());
-            HIP_TRY(hipFree(c->d_partial));
-        }
-        HIP_TRY(hipMalloc(&c->d_partial, need * sizeof(double)));
-        c->partial_cap = need;
-    }
-    hipLaunchKernelGGL(nn_energy_kernel<Real>, dim3(nbe, R), dim3(256), 0, s, dm, model, N, cache, c->d_partial);
+    double *partial = nullptr;
+    if (int rc1 = partial_for(c, s, need, &partial)) return rc1;
+    hipLaunchKernelGGL(nn_energy_kernel<Real>, dim3(nbe, R), dim3(256), 0, s, dm, model, N, cache, partial);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(nn_energy_finish_kernel, dim3(R), dim3(64), 0, s, dm, model, c->d_partial, nbe, E);
+    hipLaunchKernelGGL(nn_energy_finish_kernel, dim3(R), dim3(64), 0, s, dm, model, partial, nbe, E);
     HIP_TRY(hipGetLastError());
     return NCE_OK;
 }

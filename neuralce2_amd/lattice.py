@@ -4,7 +4,8 @@ A minimal stand-in for the parts of ``ase`` the Monte Carlo path needs (cell + p
 ``repeat``, fcc builder, cif read/write) plus the lattice arithmetic of the reference's template-only tables:
 
 * ``site_decompose``  every supercell site -> (basis index, integer primitive-cell offset), integer supercell matrix T
-  (reference: neuralce/nce/gpu_layer0_tiled.py:16-33).
+  (contract of neuralce/nce/gpu_layer0_tiled.py:16-33; bases chosen by Cartesian distance, offsets reduced exactly
+  in integers through the adjugate of T).
 * ``smith_normal_form``  U T V = diag(d) with U, V unimodular (reference: neuralce/ce/ce_gpu_implicit.py:28-39 uses
   sympy; this is an exact integer implementation with the same contract).
 * ``canonical_cells``  k = U n mod d and the cell index (k0 d1 + k1) d2 + k2 (ce_gpu_implicit.py:42-46).
@@ -59,7 +60,7 @@ class Structure:
         """Tile the cell; atom order ((m0*n1 + m1)*n2 + m2)*n_atoms + a, as ase.Atoms.repeat."""
         reps = tuple(int(r) for r in (reps if np.ndim(reps) else (reps,) * 3))
         n = len(self)
-        m = np.array([(a, b, c) for a in range(reps[0]) for b in range(reps[1]) for c in range(reps[2])], np.int64)
+        m = np.indices(reps, dtype=np.int64).reshape(3, -1).T          # (m0, m1, m2), m2 fastest
         pos = (self.positions[None, :, :] + (m @ self.cell)[:, None, :]).reshape(-1, 3)
         num = np.tile(self.numbers, len(m))
         cell = self.cell * np.array(reps, np.float64)[:, None]
@@ -143,51 +144,103 @@ def canonical_cells(off, U, d):
     return k, (k[..., 0] * d[1] + k[..., 1]) * d[2] + k[..., 2]
 
 
-def site_decompose(prim, atoms, tol=1e-3):
-    """Every supercell site -> (basis index, integer primitive-cell offset reduced into the supercell),
-    integer supercell matrix T (S = T P), n_cells and the reduction function (gpu_layer0_tiled.py:16-33)."""
-    P = prim.cell; S = atoms.cell
-    Tf = S @ np.linalg.inv(P)
-    if not np.allclose(Tf, np.round(Tf), atol=1e-4):
-        raise AssertionError(f"supercell is not an integer multiple of the primitive: T={Tf}")
-    T = np.round(Tf).astype(np.int64)
-    n_cells = int(round(abs(np.linalg.det(T))))
-    F = atoms.positions @ np.linalg.inv(P)
-    pb = prim.get_scaled_positions()
-    basis = np.full(len(atoms), -1, np.int64)
-    off = np.zeros((len(atoms), 3), np.int64)
-    best = np.full(len(atoms), np.inf)
-    for b in range(len(prim)):
-        dd = F - pb[b]; t = np.round(dd); err = np.abs(dd - t).max(1)
-        take = err < best
-        basis[take] = b; off[take] = t[take]; best[take] = err[take]
-    if best.max() >= tol:
-        raise AssertionError(f"site does not sit on the primitive lattice (max err {best.max():.2e})")
-    Tinv = np.linalg.inv(T)
+class OffLatticeError(ValueError):
+    """A site of the structure is not a lattice site of the primitive (or the cell is not an integer supercell)."""
 
-    def reduce(t):
-        f = np.asarray(t) @ Tinv
-        return np.asarray(t) - np.floor(f + 1e-9).astype(np.int64) @ T
-    return basis, reduce(off), T, n_cells, reduce
+
+def integer_supercell(prim, atoms):
+    """T with S = T P (rows: lattice vectors) and n_cells = |det T|; refuses a cell that is no integer multiple."""
+    Tf = np.linalg.solve(prim.cell.T, atoms.cell.T).T
+    T = np.rint(Tf).astype(np.int64)
+    if not np.allclose(Tf, T, atol=1e-4):
+        raise OffLatticeError(f"the cell is not an integer supercell of the primitive: T = {np.round(Tf, 4).tolist()}")
+    n_cells = abs(int(round(np.linalg.det(T))))
+    if n_cells == 0:
+        raise OffLatticeError("singular supercell matrix")
+    return T, n_cells
+
+
+def lattice_coordinates(prim, atoms, tol=5e-3):
+    """Every site x -> (basis b, integer cell vector n) with x = (n + f_b) P, f_b the basis' fractional position.
+
+    The basis is the one whose nearest lattice point is closest in Cartesian distance; a site farther than `tol`
+    Angstrom from every lattice point raises OffLatticeError.  n is not reduced into the supercell."""
+    P = prim.cell
+    frac = np.linalg.solve(P.T, atoms.positions.T).T
+    fb = prim.get_scaled_positions()
+    best = np.full(len(atoms), np.inf)
+    basis = np.zeros(len(atoms), np.int64)
+    n = np.zeros((len(atoms), 3), np.int64)
+    for b in range(len(prim)):
+        r = frac - fb[b]
+        nb = np.rint(r)
+        miss = np.linalg.norm((r - nb) @ P, axis=1)
+        take = miss < best
+        best[take] = miss[take]
+        basis[take] = b
+        n[take] = nb[take].astype(np.int64)
+    if len(atoms) and best.max() > tol:
+        i = int(best.argmax())
+        raise OffLatticeError(f"site {i} lies {best[i]:.3g} A from the nearest lattice site of the primitive")
+    return basis, n, best
+
+
+def _adjugate3(T):
+    """integer adjugate of a 3 x 3 integer matrix: T @ adj(T) = det(T) I."""
+    t = [[int(x) for x in row] for row in np.asarray(T)]
+    adj = [[0] * 3 for _ in range(3)]
+    for i in range(3):
+        for j in range(3):
+            r = [k for k in range(3) if k != j]
+            c = [k for k in range(3) if k != i]
+            minor = t[r[0]][c[0]] * t[r[1]][c[1]] - t[r[0]][c[1]] * t[r[1]][c[0]]
+            adj[i][j] = (-1) ** (i + j) * minor
+    return np.array(adj, np.int64)
+
+
+def reduce_offsets(n, T):
+    """Integer cell vectors (..., 3) -> the representative n - q T inside the supercell, q = floor(n T^-1),
+    evaluated exactly in integers as floor(n adj(T) / det T)."""
+    n = np.asarray(n, np.int64)
+    adj = _adjugate3(T)
+    det = int(round(np.linalg.det(np.asarray(T, float))))
+    num = n @ adj
+    if det < 0:
+        num, det = -num, -det
+    return n - np.floor_divide(num, det) @ np.asarray(T, np.int64)
+
+
+def site_decompose(prim, atoms, tol=5e-3):
+    """Every supercell site -> (basis index, integer primitive-cell offset reduced into the supercell), the integer
+    supercell matrix T (S = T P), n_cells and the reduction function; the contract of the reference's
+    site_decompose (nce/gpu_layer0_tiled.py:16-33).  Off-lattice sites raise OffLatticeError."""
+    T, n_cells = integer_supercell(prim, atoms)
+    basis, n, _ = lattice_coordinates(prim, atoms, tol)
+    return basis, reduce_offsets(n, T), T, n_cells, lambda t: reduce_offsets(t, T)
 
 
 class CanonicalCell:
-    """Canonical site order of a supercell of ``prim``: SNF (U, d), perm / inv_perm, basis of every site."""
+    """Canonical site order of a supercell of ``prim``: SNF (U, d), perm / inv_perm, basis of every site.
+
+    k = U n mod d is unchanged by a supercell translation n -> n + m T, so the cell index needs no reduction of the
+    site offsets first; ``index`` holds each input site's canonical index."""
 
     def __init__(self, prim, atoms):
         self.prim = prim
         self.nB = len(prim)
-        basis, off_red, T, n_cells, _ = site_decompose(prim, atoms)
+        T, n_cells = integer_supercell(prim, atoms)
+        basis, n, _ = lattice_coordinates(prim, atoms)
         N = len(atoms)
         if N != self.nB * n_cells:
-            raise AssertionError((N, self.nB, n_cells))
+            raise OffLatticeError(f"{N} sites for {n_cells} primitive cells of {self.nB} sites")
         U, d, _ = smith_normal_form(T.T)            # rows of T = supercell vectors -> lattice T^T Z^3
         if int(np.prod(d)) != n_cells:
-            raise AssertionError("SNF volume mismatch")
-        _, cell_idx = canonical_cells(off_red, U, d)
+            raise OffLatticeError("Smith normal form volume differs from |det T|")
+        _, cell_idx = canonical_cells(n, U, d)
         canon = cell_idx * self.nB + basis
-        if len(np.unique(canon)) != N:
-            raise AssertionError("sites do not form a clean tiling of the primitive")
+        if np.bincount(canon, minlength=N).max() > 1:
+            raise OffLatticeError("two sites fall on the same lattice site: not a clean tiling of the primitive")
+        self.index = canon
         self.perm = np.argsort(canon).astype(np.int64)
         self.inv_perm = np.empty(N, np.int64); self.inv_perm[self.perm] = np.arange(N)
         self.U, self.d, self.T = U, d, T

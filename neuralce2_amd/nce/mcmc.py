@@ -1,12 +1,16 @@
 """Parallel-tempering MC for the NeuralCE model (reference: neuralce/nce/mcmc.py).
 
-``_run_replicas`` follows mcmc.py:288-422: chunks of swap steps for every replica (one ``nce_nn_run_chunk`` launch),
-a full recompute of caches and energies every ``anchor_every`` chunks with the drift logged, and at the end of every
-chunk ``max(chunk // exchange_every, 1)`` rounds of neighbour configuration swaps accepted with
-d = (b_i - b_{i+1})(E_i - E_{i+1}), d >= 0 or u < exp(d) in FP64 on the host (mcmc.py:370-384; configurations and
-caches move, O(R N)).  Outputs: energies.npz (E_chunk, acc_chunk, drift, drift_eV, traj, betas, temps),
-final_sigma.npy in input atom order.  Random numbers: Philox per (seed, replica, step) on the device, numpy
-Generator(seed) for the exchange.
+``_run_replicas`` keeps the contract of mcmc.py:288-422: chunks of swap steps for every replica (one
+``nce_nn_run_chunk`` launch), a full recompute of caches and energies every ``anchor_every`` chunks (FP32 / BF16
+drift control; the drift is logged), neighbour exchanges of configurations between chunks accepted with
+d = (b_i - b_{i+1})(E_i - E_{i+1}), d >= 0 or u < exp(d) in FP64 (mcmc.py:370-384), checkpoint / resume through
+``checkpoint.npz`` (mcmc.py:322-399) and the output files energies.npz (E_chunk, acc_chunk, drift, drift_eV, traj,
+betas, temps), final_sigma.npy (input atom order) and, for cells up to CE_CNN_CIF_MAX sites, one cif per replica.
+
+Random numbers are counter-based: the device draws proposals from Philox keyed by (seed, replica, step), the host
+draws the exchange parity and uniforms from Philox keyed by (seed, exchange count).  A run's state is therefore its
+occupations, step counters and exchange count, which is what a checkpoint holds; a resumed run continues the same
+streams.
 """
 from __future__ import annotations
 
@@ -78,127 +82,283 @@ class NNJobState:
             t.copy_(t[idx])
 
 
+_CK_NAME = "checkpoint.npz"
+
+
+def _fingerprint(nn, sig):
+    """16 hex digits identifying the network parameters, the standardisation and the starting occupations."""
+    import hashlib
+    h = hashlib.sha256()
+    P = nn.P
+
+    def feed(x):
+        h.update(np.ascontiguousarray(np.asarray(x, np.float64)).tobytes())
+    feed(P["lin"]); feed(nn.mu); feed(nn.sd); feed(P["head"]["b"])
+    if "w" in P["head"]:
+        feed(P["head"]["w"])
+    for L in P["layers"]:
+        for k in sorted(L):
+            h.update(k.encode()); feed(L[k])
+    h.update(np.ascontiguousarray(np.asarray(sig), np.int8).tobytes())
+    return h.hexdigest()[:16]
+
+
+def exchange_draws(seed, ex_index, R):
+    """parity (0/1) and R uniforms of exchange number `ex_index`: Philox keyed by the run seed, counter
+    (ex_index, i, 'EX')."""
+    from ..philox import philox4x32_10, uniform53
+    key = (int(seed) & 0xFFFFFFFF, ((int(seed) >> 32) & 0xFFFFFFFF) ^ 0x45584348)
+    lo, hi = int(ex_index) & 0xFFFFFFFF, (int(ex_index) >> 32) & 0xFFFFFFFF
+    ctr = np.zeros((R + 1, 4), np.uint64)
+    ctr[:, 0] = lo; ctr[:, 1] = hi; ctr[:, 2] = np.arange(R + 1); ctr[:, 3] = 0x4558
+    w = philox4x32_10(ctr, key)
+    return int(w[R, 0] & 1), uniform53(w[:R])
+
+
+def ladder_swaps(E_total, betas, parity, u):
+    """neighbour pairs (i, i+1) from `parity` on, accepted when d = (b_i - b_{i+1})(E_i - E_{i+1}) >= 0 or
+    u_i < exp(d) (reference mcmc.py:370-384).  Returns the new order of the replicas and the (tried, accepted)
+    counts."""
+    R = len(E_total)
+    order = np.arange(R)
+    tried = acc = 0
+    for i in range(parity, R - 1, 2):
+        tried += 1
+        d = (betas[i] - betas[i + 1]) * (E_total[i] - E_total[i + 1])
+        if d >= 0 or u[i] < np.exp(d):
+            order[i], order[i + 1] = order[i + 1], order[i]
+            acc += 1
+    return order, tried, acc
+
+
+class _Record:
+    """what a run accumulates chunk by chunk (the energies.npz fields) plus the counters a resume needs."""
+
+    def __init__(self, R):
+        self.E_chunk, self.acc_chunk, self.drift_log, self.traj = [], [], [], []
+        self.n_ex = self.n_ex_try = self.n_ex_acc = 0
+        self.drift = 0.0
+        self.acc_prev = np.zeros(R, np.int64)
+
+    def traj_array(self, R):
+        return np.concatenate(self.traj, axis=1) if self.traj else np.zeros((R, 0))
+
+
+def _save_checkpoint(path, c_done, st, rec, want, betas):
+    """reference keys (mcmc.py:389-399): chunk, sig, spin, key, drift, n_ex_try, n_ex_acc, E_chunk, acc_chunk,
+    drift_log, traj, snaps, snap_chunks, cfg_betas and cfg_*; `key` holds this build's stream state (the replicas'
+    step counters and the exchange count) and n_accept the accepted-move counters."""
+    R = st.R
+    tmp = path + ".tmp.npz"
+    np.savez(tmp, chunk=np.int64(c_done), sig=st.occ.cpu().numpy(), spin=np.zeros(0),
+             key=np.concatenate([st.step.cpu().numpy(), [rec.n_ex]]).astype(np.int64),
+             n_accept=st.n_accept.cpu().numpy(), drift=np.float64(rec.drift), n_ex_try=np.int64(rec.n_ex_try),
+             n_ex_acc=np.int64(rec.n_ex_acc), E_chunk=np.array(rec.E_chunk).reshape(-1, R),
+             acc_chunk=np.array(rec.acc_chunk).reshape(-1, R), drift_log=np.array(rec.drift_log, np.float64),
+             traj=rec.traj_array(R), snaps=np.zeros((0, R, st.nn.N), np.int8), snap_chunks=np.zeros(0, np.int64),
+             cfg_betas=np.asarray(betas, np.float64), **{"cfg_" + k: v for k, v in want.items()})
+    os.replace(tmp, path)
+
+
+def _load_checkpoint(path, want, betas):
+    ck = np.load(path, allow_pickle=False)
+    have = {k: (str(ck["cfg_" + k]) if isinstance(v, str) else int(ck["cfg_" + k])) for k, v in want.items()
+            if "cfg_" + k in ck.files}
+    same_T = np.allclose(np.asarray(ck["cfg_betas"], np.float64), np.asarray(betas, np.float64), rtol=0, atol=1e-6)
+    if have != want or not same_T:
+        diff = sorted(k for k in want if have.get(k) != want[k]) + ([] if same_T else ["temperatures"])
+        raise RuntimeError(f"{path} was written by another run (it differs in {', '.join(diff)}); remove the file "
+                           f"to start this run from the beginning")
+    return ck
+
+
 def _run_replicas(mc, betas, n_steps, seed, chunk, log_every, sample_interval, out_dir, tag, exchange_every=0,
                   anchor_every=None, init_npz=None, checkpoint_every=0):
+    """R replicas of the NeuralCE model, replica r at betas[r]; neighbour exchanges when exchange_every > 0.
+    anchor_every: chunks between full recomputes (default ceil(N / chunk), about one per sweep).
+    init_npz: starting occupations (R, N) in canonical order (default: the setup's configuration everywhere).
+    checkpoint_every: chunks between checkpoints, rounded up to a multiple of anchor_every so a checkpoint follows a
+    full recompute and a resumed run repeats the uninterrupted one."""
     nn = mc["nn"]
     R = len(betas)
     betas = np.asarray(betas, np.float32).astype(np.float64)          # the reference keeps beta in float32
-    sig0 = mc["sig0"] if init_npz is None else init_npz
-    n_chunks = max(n_steps // chunk, 1)
     if chunk % sample_interval != 0:
         raise ValueError(f"chunk ({chunk}) must be a multiple of sample_interval ({sample_interval})")
-    st = NNJobState(nn, sig0, betas, seed=seed, precision=mc["precision"], sample_interval=sample_interval,
-                    samples_per_chunk=chunk // sample_interval)
+    n_chunks = max(n_steps // chunk, 1)
     if anchor_every is None:
         anchor_every = max(1, int(np.ceil(nn.N / chunk)))
-    rng = np.random.default_rng(seed)
-    E_chunk, acc_chunk, drift_log, traj = [], [], [], []
-    n_ex_try = n_ex_acc = 0
-    drift = 0.0
-    _log(f"[{tag}] {R} replicas, {n_chunks} chunks of {chunk} steps; full re-anchor every {anchor_every} chunk(s)")
+    sig0 = np.broadcast_to(np.asarray(mc["sig0"], np.int8), (R, nn.N)) if init_npz is None else np.asarray(init_npz)
+    ck_every = int(checkpoint_every or 0)
+    if ck_every:
+        ck_every = -(-ck_every // anchor_every) * anchor_every
+    ck_path = os.path.join(out_dir, _CK_NAME)
+    want = dict(n_steps=int(n_steps), chunk=int(chunk), R=int(R), n_atoms=int(nn.N),
+                sample_interval=int(sample_interval), seed=int(seed), anchor_every=int(anchor_every),
+                exchange_every=int(exchange_every), model=_fingerprint(nn, sig0))
+    rec = _Record(R)
+    c_first = 0
+    resume = None
+    if ck_every and os.path.exists(ck_path):
+        resume = _load_checkpoint(ck_path, want, betas)
+        c_first = int(resume["chunk"])
+        sig0 = resume["sig"]
+        rec.E_chunk = list(resume["E_chunk"]); rec.acc_chunk = list(resume["acc_chunk"])
+        rec.drift_log = [float(x) for x in resume["drift_log"]]
+        rec.traj = [resume["traj"]] if resume["traj"].shape[1] else []
+        rec.n_ex = int(resume["key"][-1]); rec.n_ex_try = int(resume["n_ex_try"])
+        rec.n_ex_acc = int(resume["n_ex_acc"]); rec.drift = float(resume["drift"])
+    st = NNJobState(nn, sig0, betas, seed=seed, precision=mc["precision"], sample_interval=sample_interval,
+                    samples_per_chunk=chunk // sample_interval)
+    if resume is not None:
+        import torch
+        st.step.copy_(torch.as_tensor(resume["key"][:-1], dtype=torch.int64))
+        st.n_accept.copy_(torch.as_tensor(resume["n_accept"], dtype=torch.int64))
+        rec.acc_prev = resume["n_accept"].astype(np.int64)
+        _log(f"[{tag}] continuing from {ck_path} after chunk {c_first} of {n_chunks}")
+    ex_rounds = max(chunk // exchange_every, 1) if (exchange_every > 0 and R > 1) else 0
+    _log(f"[{tag}] {R} replicas x {n_chunks} chunks of {chunk} steps on {nn.N} sites; caches rebuilt every "
+         f"{anchor_every} chunk(s)" + (f", {ex_rounds} exchange round(s) per chunk" if ex_rounds else "")
+         + (f", checkpoint every {ck_every} chunks" if ck_every else ""))
+    log_chunks = max(log_every // chunk, 1) if log_every else 0
     t0 = time.time()
-    acc_prev = np.zeros(R, np.int64)
-    for c in range(n_chunks):
+    for c in range(c_first, n_chunks):
         st.run(chunk)
-        if (c + 1) % anchor_every == 0 or c == n_chunks - 1:
-            E_old, E_new = st.anchor()
-            drift = float((E_new - E_old).abs().max().item())
-        E = st.energy.cpu().numpy()
-        E_eV = E * nn.ys + nn.ym
-        acc = st.n_accept.cpu().numpy()
-        E_chunk.append(E_eV); acc_chunk.append((acc - acc_prev) / chunk); acc_prev = acc
-        drift_log.append(drift)
-        traj.append(st.traj.cpu().numpy().T * nn.ys + nn.ym)
-        if exchange_every > 0 and R > 1:
-            for _ in range(max(chunk // exchange_every, 1)):
-                start = int(rng.integers(0, 2))
-                u = rng.random(R)
-                E_np = st.energy.cpu().numpy() * nn.ys * nn.N
-                order = list(range(R))
-                for i in range(start, R - 1, 2):
-                    n_ex_try += 1
-                    d = (betas[i] - betas[i + 1]) * (E_np[i] - E_np[i + 1])
-                    if d >= 0 or u[i] < np.exp(d):
-                        order[i], order[i + 1] = order[i + 1], order[i]; n_ex_acc += 1
-                if order != list(range(R)):
-                    st.permute(order)
-        if log_every and ((c + 1) % max(log_every // chunk, 1) == 0 or c == n_chunks - 1):
-            _log(f"[{tag}] chunk {c+1}/{n_chunks}  E(eV/atom) mean {E_eV.mean():.6f} [{E_eV.min():.6f},"
-                 f"{E_eV.max():.6f}]  acc {np.mean(acc_chunk[-1]):.3f}  drift {drift:.1e}  "
-                 f"{R * chunk * (c + 1) / (time.time() - t0):,.0f} steps/s" + (f"  exch {n_ex_acc}/{n_ex_try}"
-                                                                              if exchange_every else ""))
+        last = c == n_chunks - 1
+        if (c + 1) % anchor_every == 0 or last:
+            E_run, E_full = st.anchor()
+            rec.drift = float((E_full - E_run).abs().max().item())
+        E_eV = st.energy.cpu().numpy() * nn.ys + nn.ym
+        n_acc = st.n_accept.cpu().numpy()
+        rec.E_chunk.append(E_eV)
+        rec.acc_chunk.append((n_acc - rec.acc_prev) / chunk)
+        rec.acc_prev = n_acc
+        rec.drift_log.append(rec.drift)
+        rec.traj.append(st.traj.cpu().numpy().T * nn.ys + nn.ym)
+        for _ in range(ex_rounds):
+            parity, u = exchange_draws(seed, rec.n_ex, R)
+            rec.n_ex += 1
+            order, tried, acc = ladder_swaps(st.energy.cpu().numpy() * nn.de_scale, betas, parity, u)
+            rec.n_ex_try += tried; rec.n_ex_acc += acc
+            if acc:
+                st.permute(order)
+        if log_chunks and ((c + 1) % log_chunks == 0 or last):
+            rate = R * chunk * (c + 1 - c_first) / max(time.time() - t0, 1e-9)
+            _log(f"[{tag}] {c + 1}/{n_chunks}: E {E_eV.mean():.6f} eV/atom ({E_eV.min():.6f} .. {E_eV.max():.6f}), "
+                 f"accepted {np.mean(rec.acc_chunk[-1]):.3f}, rebuild drift {rec.drift:.1e}, {rate:,.0f} attempts/s"
+                 + (f", exchanges {rec.n_ex_acc}/{rec.n_ex_try}" if ex_rounds else ""))
+        if ck_every and (c + 1) % ck_every == 0 and not last:
+            os.makedirs(out_dir, exist_ok=True)
+            rec.traj = [rec.traj_array(R)]
+            _save_checkpoint(ck_path, c + 1, st, rec, want, betas)
     wall = time.time() - t0
     os.makedirs(out_dir, exist_ok=True)
-    E_chunk = np.array(E_chunk); acc_chunk = np.array(acc_chunk); traj = np.concatenate(traj, axis=1)
-    np.savez(f"{out_dir}/energies.npz", E_chunk=E_chunk, acc_chunk=acc_chunk, drift=np.array(drift_log),
-             drift_eV=np.array(drift_log) * nn.ys, traj=traj, betas=betas, temps=1.0 / (KB * betas))
-    sig_np = st.occ.cpu().numpy()[:, mc["inv_perm"]]
-    np.save(f"{out_dir}/final_sigma.npy", sig_np)
-    return dict(steps_per_s=float(R * chunk * n_chunks / wall), n_replicas=R, n_steps=int(n_steps), chunk=int(chunk),
-                E_final=[float(x) for x in E_chunk[-1]], acc_mean=float(acc_chunk.mean()),
-                drift_max=float(np.max(drift_log)), exchange_acc=(n_ex_acc / n_ex_try if n_ex_try else None),
-                model_depth=int(mc["depth"]), n_atoms=int(nn.N))
+    E_chunk = np.array(rec.E_chunk); acc_chunk = np.array(rec.acc_chunk); drift = np.array(rec.drift_log)
+    np.savez(os.path.join(out_dir, "energies.npz"), E_chunk=E_chunk, acc_chunk=acc_chunk, drift=drift,
+             drift_eV=drift * nn.ys, traj=rec.traj_array(R), betas=betas, temps=1.0 / (KB * betas))
+    sig_in = st.occ.cpu().numpy()[:, mc["inv_perm"]]
+    np.save(os.path.join(out_dir, "final_sigma.npy"), sig_in)
+    if nn.N <= int(os.environ.get("CE_CNN_CIF_MAX", "200000")) and mc.get("atoms") is not None:
+        _write_cifs(mc, sig_in, 1.0 / (KB * betas), out_dir)
+    if ck_every and os.path.exists(ck_path):
+        os.remove(ck_path)
+    return dict(steps_per_s=float(R * chunk * (n_chunks - c_first) / max(wall, 1e-9)), n_replicas=R,
+                n_steps=int(n_steps), chunk=int(chunk), E_final=[float(x) for x in E_chunk[-1]],
+                acc_mean=float(acc_chunk.mean()), drift_max=float(np.max(drift)),
+                exchange_acc=(rec.n_ex_acc / rec.n_ex_try if rec.n_ex_try else None), model_depth=int(mc["depth"]),
+                n_atoms=int(nn.N), resumed_from_chunk=int(c_first))
+
+
+def _write_cifs(mc, sig_in, temps, out_dir):
+    """final_T<T>_r<r>.cif per replica: the input structure with its active sites decorated by the final species."""
+    from ..lattice import ATOMIC_NUMBERS, write_cif
+    Z_of = np.array([ATOMIC_NUMBERS[s] for s in mc["nn"].geo["species"]], np.int64)
+    for r in range(sig_in.shape[0]):
+        at = mc["atoms"].copy()
+        at.numbers = Z_of[np.asarray(sig_in[r], np.int64)]
+        write_cif(os.path.join(out_dir, f"final_T{temps[r]:.0f}_r{r}.cif"), at)
 
 
 def _prepare(model_path, init_cif, supercell=(1, 1, 1), seed_idx=0, dtype="float32", seed=0, **_):
+    """model + starting cell -> the setup() dict (reference mcmc.py _prepare): the cif is tiled by `supercell`; the
+    tiled decoration is shuffled with the run seed unless CE_CNN_SUPERCELL_PERIODIC=1 keeps it periodic."""
     from ..lattice import read_cif
     from .mc_model import load_model, setup
     mdl = load_model(model_path)
     atoms = read_cif(init_cif)
-    sc = tuple(int(x) for x in supercell)
-    if sc != (1, 1, 1):
-        atoms = atoms.repeat(sc)
+    reps = tuple(int(x) for x in supercell)
+    if reps != (1, 1, 1):
+        atoms = atoms.repeat(reps)
         if os.environ.get("CE_CNN_SUPERCELL_PERIODIC", "0") != "1":
-            Z = atoms.numbers.copy()
-            atoms.numbers = np.random.default_rng(seed).permutation(Z)
+            atoms.numbers = np.random.default_rng(seed).permutation(atoms.numbers)
     t0 = time.time()
     mc = setup(mdl, atoms, seed_idx=seed_idx, dtype=dtype)
-    _log(f"[ce_cnn] model {os.path.basename(os.path.dirname(model_path))}: depth {mc['depth']}, {mc['n_atoms']} atoms, "
-         f"setup {time.time()-t0:.1f}s, E0 = {mc['energy'](mc['sig0']):.6f} eV/atom")
-    mc["mdl"] = mdl; mc["atoms"] = atoms
+    mc["mdl"] = mdl
+    mc["atoms"] = atoms
+    _log(f"NeuralCE {os.path.basename(os.path.dirname(os.path.abspath(model_path)))}: {mc['n_atoms']} sites, "
+         f"{mc['depth']} hidden layer(s), tables built in {time.time() - t0:.1f} s; starting energy "
+         f"{mc['energy'](mc['sig0']):.6f} eV/atom")
     return mc
+
+
+def _start_from_npz(mc, path, temps):
+    """per-replica starting occupations (R, N) in canonical order from an --init_npz file."""
+    from ..lattice import ATOMIC_NUMBERS
+    from ..mc.init_structures import load_init_npz
+    numbers, _ = load_init_npz(path, mc["mdl"]["primitive"], mc["atoms"], temps)
+    index_of = {ATOMIC_NUMBERS[s]: k for k, s in enumerate(mc["nn"].geo["species"])}
+    bad = sorted(set(np.unique(numbers).tolist()) - set(index_of))
+    if bad:
+        raise ValueError(f"{path}: atomic numbers {bad} are not species of the model")
+    lut = np.zeros(256, np.int8)
+    for z, k in index_of.items():
+        lut[z] = k
+    return lut[numbers.astype(np.int64)][:, mc["perm"]]
+
+
+def _drive(mc, temps, tag, n_steps, seed, chunk, log_every, sample_interval, out_dir, exchange_every=0,
+           anchor_every=None, init_npz=None, checkpoint_every=0):
+    temps = np.asarray(temps, float)
+    start = _start_from_npz(mc, init_npz, temps) if init_npz else None
+    if init_npz:
+        _log(f"[{tag}] replica starting structures from {init_npz}")
+    return _run_replicas(mc, 1.0 / (KB * temps), n_steps, seed, min(chunk, n_steps), log_every, sample_interval,
+                         out_dir, tag, exchange_every=exchange_every, anchor_every=anchor_every, init_npz=start,
+                         checkpoint_every=checkpoint_every)
 
 
 def run_pt_mc(model_path, init_cif, n_steps=10000, T_min=100.0, T_max=2000.0, n_replicas=32, T_grid=None,
               exchange_every=100, seed=0, log_every=0, out_dir=".", chunk=1000, sample_interval=100,
               supercell=(1, 1, 1), seed_idx=0, dtype="float32", anchor_every=None, init_npz=None,
               checkpoint_every=0, **_):
-    """reference: neuralce/nce/mcmc.py:466-475 (geomspace ladder, beta = 1 / (kB T), kB = 8.617333262e-5)."""
+    """parallel tempering on a geometric ladder (or T_grid); beta = 1 / (kB T), kB = 8.617333262e-5 (reference
+    mcmc.py:466-475)."""
     mc = _prepare(model_path, init_cif, supercell, seed_idx, dtype, seed=seed)
     temps = np.asarray(T_grid, float) if T_grid is not None else np.geomspace(T_min, T_max, n_replicas)
-    betas = [1.0 / (KB * t) for t in temps]
-    sig0 = None
-    if init_npz:
-        from ..mc.init_structures import load_init_npz
-        from ..lattice import ATOMIC_NUMBERS
-        nums, _ = load_init_npz(init_npz, mc["mdl"]["primitive"], mc["atoms"], temps)
-        lut = np.full(256, -1, np.int64)
-        for k, sp in enumerate(mc["nn"].geo["species"]):
-            lut[ATOMIC_NUMBERS[sp]] = k
-        sig = lut[nums.astype(np.int64)]
-        if (sig < 0).any():
-            raise ValueError("init_npz holds a species outside the model's")
-        sig0 = sig[:, mc["perm"]].astype(np.int8)
-    res = _run_replicas(mc, betas, n_steps, seed, min(chunk, n_steps), log_every, sample_interval, out_dir,
-                        "ce_cnn PT", exchange_every=exchange_every, anchor_every=anchor_every, init_npz=sig0)
+    res = _drive(mc, temps, "nce PT", n_steps, seed, chunk, log_every, sample_interval, out_dir,
+                 exchange_every=exchange_every, anchor_every=anchor_every, init_npz=init_npz,
+                 checkpoint_every=checkpoint_every)
     res.update(temps=[float(t) for t in temps], exchange_every=int(exchange_every))
     return res
 
 
 def run_mc(model_path, init_cif, n_steps=10000, temperature=300.0, n_replicas=1, chunk=1000, sample_interval=100,
            seed=0, log_every=0, out_dir=".", supercell=(1, 1, 1), seed_idx=0, dtype="float32", anchor_every=None,
-           **_):
+           init_npz=None, checkpoint_every=0, **_):
+    """n_replicas independent chains at one temperature."""
     mc = _prepare(model_path, init_cif, supercell, seed_idx, dtype, seed=seed)
-    res = _run_replicas(mc, [1.0 / (KB * temperature)] * n_replicas, n_steps, seed, min(chunk, n_steps), log_every,
-                        sample_interval, out_dir, f"ce_cnn T={temperature:.0f}", anchor_every=anchor_every)
+    res = _drive(mc, [float(temperature)] * int(n_replicas), f"nce T={temperature:.0f}", n_steps, seed, chunk,
+                 log_every, sample_interval, out_dir, anchor_every=anchor_every, init_npz=init_npz,
+                 checkpoint_every=checkpoint_every)
     res.update(temperature=float(temperature))
     return res
 
 
 def run_multiT_mc(model_path, init_cif, temps, n_steps=10000, chunk=1000, sample_interval=100, seed=0, log_every=0,
-                  out_dir=".", supercell=(1, 1, 1), seed_idx=0, dtype="float32", anchor_every=None, **_):
+                  out_dir=".", supercell=(1, 1, 1), seed_idx=0, dtype="float32", anchor_every=None, init_npz=None,
+                  checkpoint_every=0, **_):
+    """one independent chain per temperature, no exchange."""
     mc = _prepare(model_path, init_cif, supercell, seed_idx, dtype, seed=seed)
-    res = _run_replicas(mc, [1.0 / (KB * t) for t in temps], n_steps, seed, min(chunk, n_steps), log_every,
-                        sample_interval, out_dir, "ce_cnn multiT", anchor_every=anchor_every)
+    res = _drive(mc, temps, "nce multiT", n_steps, seed, chunk, log_every, sample_interval, out_dir,
+                 anchor_every=anchor_every, init_npz=init_npz, checkpoint_every=checkpoint_every)
     res.update(temps=[float(t) for t in temps])
     return res

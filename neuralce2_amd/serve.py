@@ -7,12 +7,16 @@ continuous batching at chunk granularity:
   * admission: a request that has arrived is admitted when its device memory (occupations, caches, counters) fits in
     the budget (default: the smaller of 288 GB and 90 % of the free device memory); requests are taken in arrival
     order;
-  * one tick = one chunk of `chunk` steps of every admitted cluster-expansion job in ONE nce_pt_run_chunk launch
-    (all (job, replica) pairs; the speculative kernel handles jobs of different cells together) and one chunk of
-    every admitted NeuralCE job in one nce_nn_run_chunk launch on a second stream, then the temperature-swap
-    exchange of every CE job in one nce_pt_exchange launch;
-  * a job that has done its steps retires: its per-slot <E> and <E^2> (FP64, every production step) give E(T) and
-    C_v(T), its memory is released, and waiting requests are admitted before the next tick.
+  * one cluster-expansion tick = one chunk of `chunk` steps of every admitted CE job in ONE chunk launch (all
+    (job, replica) pairs; the speculative kernel handles jobs of different cells together), the FP64 moment merge
+    and the temperature-swap exchange of every CE job, captured once as a HIP graph (nce_pt_cycle) for the current
+    set of jobs and replayed each tick with no host work in between; the host runs at most two ticks ahead of the
+    device and rebuilds the graph only when a job joins, leaves or ends its equilibration;
+  * the NeuralCE jobs run one nce_nn_run_chunk launch on a second stream, left in flight while the CE ticks
+    continue; their caches are rebuilt every ceil(N / chunk) chunks (FP32 / BF16 drift) before the exchange;
+  * a job that has done its steps retires: its per-slot <E> and <E^2> (FP64, every production step after its
+    n_equilib equilibration steps) give E(T) and C_v(T), its memory is released, and waiting requests are admitted
+    before the next tick.
 
 Random numbers are Philox keyed by (job seed, replica, step) and the exchange by (job seed, exchange count), so a
 cluster-expansion job's trajectory does not depend on which other jobs share its launches: a job gives the same result
@@ -45,6 +49,7 @@ class JobSpec:
     seed: int = 0
     arrival: float = 0.0        # seconds after the start of the stream
     kind: str = "ce"            # "ce" (synthetic 8/6/5 cluster expansion) | "nce" (NeuralCE model of the server)
+    n_equilib: int = 0          # leading steps left out of the moments (a whole number of chunks)
 
 
 @dataclass
@@ -61,6 +66,7 @@ class JobResult:
     admitted: float
     finished: float
     occ: Optional[np.ndarray] = None   # final occupations by temperature slot (canonical order), on request
+    drift: float = 0.0          # largest |running total - full recompute| at retirement (eV per site)
 
     @property
     def latency(self):
@@ -89,7 +95,8 @@ class _CEJob:
         self.state = PTJobState(gce, _initial_sigma(self.N, spec.x, spec.seed), betas, seed=spec.seed,
                                 sample_interval=chunk, samples_per_chunk=1, job_id=spec.job_id)
         self.steps_done = 0
-        self.n_ex = 0
+        self.equilibrating = spec.n_equilib > 0
+        self.drift = 0.0
 
     @staticmethod
     def bytes_needed(spec):
@@ -97,12 +104,16 @@ class _CEJob:
         return spec.n_temps * (N + 128) + 4096
 
     def result(self, keep_occ):
+        import torch
         st = self.state
         mean, var = st.moments()                      # per slot, total energies
         T = self.temps.astype(np.float64)
         E = mean / self.N
         cv = var / (KB_CE * T) ** 2 / self.N
         acc = float(st.n_accept.sum().item()) / max(1, self.steps_done * st.R)
+        full = torch.empty_like(st.energy)
+        st.gce.ctx.ce_energy(st.gce.model, st.occ, full)
+        self.drift = float((full - st.energy).abs().max().item()) / self.N
         occ = None
         if keep_occ:
             rep = st.rep_of_slot.cpu().numpy()
@@ -111,21 +122,26 @@ class _CEJob:
 
 
 class _NCEJob:
-    """NeuralCE job: FP32 (or BF16-MFMA) local updates; the exchange permutes configurations on the host
-    (neuralce/nce/mcmc.py:370-384), every chunk."""
+    """NeuralCE job: FP32 (or BF16-MFMA) local updates; caches and energies rebuilt every ceil(N / chunk) chunks
+    (reference mcmc.py:360-366); the exchange permutes configurations on the host every chunk (mcmc.py:370-384)
+    with draws from Philox keyed by (job seed, exchange count)."""
 
     def __init__(self, spec, nn, chunk, precision):
         from .nce.mcmc import NNJobState
         self.spec = spec
         self.nn = nn
+        self.chunk = chunk
         self.temps, betas = _ladder(spec)
         self.betas = (1.0 / (8.617333262e-5 * self.temps.astype(np.float64))).astype(np.float32).astype(np.float64)
         self.N = nn.N
         self.state = NNJobState(nn, _initial_sigma(self.N, spec.x, spec.seed), self.betas, seed=spec.seed,
                                 precision=precision, sample_interval=1, samples_per_chunk=chunk)
-        self.rng = np.random.default_rng(spec.seed)
+        self.anchor_every = max(1, int(math.ceil(self.N / chunk)))
+        self.n_chunks = 0
+        self.n_ex = 0
         self.steps_done = 0
         self.acc_n = 0
+        self.drift = 0.0
         self.s1 = np.zeros(spec.n_temps)
         self.s2 = np.zeros(spec.n_temps)
 
@@ -134,22 +150,25 @@ class _NCEJob:
         el = 8 if precision == 0 else 4
         return spec.n_temps * (nn.N + (nn.cache_elems + nn.scratch_elems) * el + 128) + 4096
 
-    def after_chunk(self, chunk):
-        tr = self.state.traj.cpu().numpy()            # (chunk, R) model units, replica = temperature
-        e = tr * self.nn.ys + self.nn.ym if self.nn.readout == "mean" else tr / self.N
-        self.s1 += e.sum(0)
-        self.s2 += (e * e).sum(0)
-        self.acc_n += e.shape[0]
+    def after_chunk(self):
+        """moments of the chunk's trajectory (after equilibration), the periodic rebuild, then the exchange."""
+        from .nce.mcmc import exchange_draws, ladder_swaps
+        self.n_chunks += 1
+        self.steps_done += self.chunk
+        if self.steps_done > self.spec.n_equilib:
+            tr = self.state.traj.cpu().numpy()        # (chunk, R) model units, replica = temperature
+            e = tr * self.nn.ys + self.nn.ym if self.nn.readout == "mean" else tr / self.N
+            self.s1 += e.sum(0)
+            self.s2 += (e * e).sum(0)
+            self.acc_n += e.shape[0]
+        if self.n_chunks % self.anchor_every == 0 or self.steps_done >= self.spec.n_steps:
+            E_run, E_full = self.state.anchor()
+            self.drift = max(self.drift, float((E_full - E_run).abs().max().item()) * self.nn.de_scale / self.N)
         R = self.state.R
-        start = int(self.rng.integers(0, 2))
-        u = self.rng.random(R)
-        E_tot = self.state.energy.cpu().numpy() * self.nn.de_scale
-        order = list(range(R))
-        for i in range(start, R - 1, 2):
-            d = (self.betas[i] - self.betas[i + 1]) * (E_tot[i] - E_tot[i + 1])
-            if d >= 0 or u[i] < math.exp(min(d, 0.0)):
-                order[i], order[i + 1] = order[i + 1], order[i]
-        if order != list(range(R)):
+        parity, u = exchange_draws(self.spec.seed, self.n_ex, R)
+        self.n_ex += 1
+        order, _, acc = ladder_swaps(self.state.energy.cpu().numpy() * self.nn.de_scale, self.betas, parity, u)
+        if acc:
             self.state.permute(order)
 
     def result(self, keep_occ):
@@ -168,6 +187,7 @@ def autotune_tile(ctx, states, precision, candidates=(4, 8, 16), n_steps=256):
     one short chunk per candidate on copies of the jobs' state (the jobs themselves do not move)."""
     import torch
     from .ce.pt_mcmc_ce import PTJobState
+    torch.cuda.synchronize()                                      # the states may still be written on their stream
     copies = []
     for st in states:
         c = PTJobState.__new__(PTJobState)
@@ -255,11 +275,13 @@ class Server:
     def run(self, specs: List[JobSpec], log=None):
         """Serve the request stream; returns (results sorted by job id, stats)."""
         import torch
+        from collections import deque
         pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
         for s in pending:
             ch = self.chunk if s.kind == "ce" else self.nn_chunk
-            if s.n_steps % ch:
-                raise ValueError(f"job {s.job_id}: n_steps {s.n_steps} is not a multiple of the chunk {ch}")
+            if s.n_steps % ch or s.n_equilib % ch or s.n_equilib >= s.n_steps:
+                raise ValueError(f"job {s.job_id}: n_steps {s.n_steps} and n_equilib {s.n_equilib} must be whole "
+                                 f"chunks of {ch}, with production steps left")
             if self._bytes(s) > self.mem_budget:
                 raise ValueError(f"job {s.job_id} needs {self._bytes(s)} bytes, more than the budget")
         active_ce, active_nn = [], []
@@ -269,24 +291,34 @@ class Server:
         t0 = time.perf_counter()
         n_ticks = 0
         nn_inflight = None                  # event after the NeuralCE chunk in flight on its stream
+        ce_inflight = deque()               # (event, jobs that finish with this tick)
+        cycle = None                        # captured tick of the current CE set
+        old_cycles = []                     # destroyed once no launch is in flight (their frees would wait on it)
+        lookahead = 2                       # CE ticks the host may run ahead of the device
 
-        def retire(group, stream):
+        def finish(job, stream):
             nonlocal mem_used
-            done = [j for j in group if j.steps_done >= j.spec.n_steps]
-            for j in done:
-                with torch.cuda.stream(stream):
-                    E, cv, acc, occ = j.result(self.keep_occ)
-                fin = time.perf_counter() - t0
-                results.append(JobResult(j.spec.job_id, j.spec.kind, j.N, j.temps.astype(np.float64), E, cv,
-                                         j.steps_done * j.state.R, acc, j.spec.arrival, j.admitted, fin, occ))
-                mem_used -= j.mem
-                group.remove(j)
-                if log:
-                    log(f"job {j.spec.job_id} ({j.spec.kind}, {j.N} sites, {j.state.R} T) done at {fin:.2f} s, "
-                        f"latency {fin - j.spec.arrival:.2f} s")
+            with torch.cuda.stream(stream):
+                E, cv, acc, occ = job.result(self.keep_occ)
+            fin = time.perf_counter() - t0
+            results.append(JobResult(job.spec.job_id, job.spec.kind, job.N, job.temps.astype(np.float64), E, cv,
+                                     job.steps_done * job.state.R, acc, job.spec.arrival, job.admitted, fin, occ,
+                                     job.drift))
+            mem_used -= job.mem
+            if log:
+                log(f"job {job.spec.job_id} ({job.spec.kind}, {job.N} sites, {job.state.R} T) done at {fin:.2f} s, "
+                    f"latency {fin - job.spec.arrival:.2f} s")
 
-        while pending or active_ce or active_nn:
+        def drain_ce(keep):
+            while len(ce_inflight) > keep:
+                ev, done = ce_inflight.popleft()
+                ev.synchronize()
+                for j in done:
+                    finish(j, self.stream_ce)
+
+        while pending or active_ce or active_nn or ce_inflight:
             now = time.perf_counter() - t0
+            changed = False
             # admission in arrival order while the memory fits; a NeuralCE job joins at the next NeuralCE launch
             while pending and pending[0].arrival <= now and mem_used + self._bytes(pending[0]) <= self.mem_budget:
                 spec = pending[0]
@@ -297,50 +329,80 @@ class Server:
                 with torch.cuda.stream(stream):
                     job = (_CEJob(spec, self._gce(spec.n), self.chunk) if spec.kind == "ce"
                            else _NCEJob(spec, self._nn(spec.n), self.nn_chunk, self.nn_precision))
-                stream.synchronize()
                 job.admitted = time.perf_counter() - t0
                 job.mem = self._bytes(spec)
                 mem_used += job.mem
-                (active_ce if spec.kind == "ce" else active_nn).append(job)
-            if not active_ce and not active_nn:
+                if spec.kind == "ce":
+                    active_ce.append(job)
+                    changed = True
+                else:
+                    active_nn.append(job)
+            if not active_ce and not active_nn and not ce_inflight:
                 if pending:
                     time.sleep(max(0.0, min(0.05, pending[0].arrival - (time.perf_counter() - t0))))
                 continue
-            # the NeuralCE jobs: one launch on their stream, left running while the CE jobs tick
+            # the NeuralCE jobs: one launch on their stream, left running while the CE ticks continue
             if nn_inflight is None and active_nn:
                 with torch.cuda.stream(self.stream_nn):
                     self.ctx.nn_run_chunk([j.state.job() for j in active_nn], self.nn_chunk, self.nn_precision)
                     nn_inflight = torch.cuda.Event()
                     nn_inflight.record(self.stream_nn)
-            # one CE tick: every CE job in one launch, then the moments, the exchange and the retirements
+            # one CE tick: the captured cycle of the current set (chunk, moment merge, exchange)
             if active_ce:
-                tile = self._tile_for(active_ce)
-                with torch.cuda.stream(self.stream_ce):
-                    self.ctx.pt_run_chunk([j.state.job(accumulate=True) for j in active_ce], self.chunk,
-                                          self.precision, tile)
-                    for j in active_ce:
-                        j.state.merge_accumulators(self.chunk, j.state.slot_of_rep.cpu().numpy())
-                        j.steps_done += self.chunk
-                        attempts += self.chunk * j.state.R
-                    self.ctx.pt_exchange([j.state.exchange_job(j.n_ex + 1, reorder_last=True) for j in active_ce])
-                self.stream_ce.synchronize()
-                for j in active_ce:
-                    j.n_ex += 1
+                if changed or cycle is None:
+                    if cycle is not None:
+                        old_cycles.append(cycle)
+                    tile = self._tile_for(active_ce)
+                    with torch.cuda.stream(self.stream_ce):
+                        cycle = self.ctx.pt_cycle(
+                            [j.state.job(accumulate=not j.equilibrating) for j in active_ce], self.chunk, 1,
+                            self.precision, tile, [j.state.exchange_job(None, reorder_last=True) for j in active_ce])
+                cycle.launch(1, self.stream_ce)
                 n_ticks += 1
-                retire(active_ce, self.stream_ce)
+                ended, rebuild = [], False
+                for j in active_ce:
+                    j.steps_done += self.chunk
+                    attempts += self.chunk * j.state.R
+                    if j.steps_done >= j.spec.n_steps:
+                        ended.append(j)
+                    elif j.equilibrating and j.steps_done >= j.spec.n_equilib:
+                        j.equilibrating = False         # moments start from this tick's end
+                        with torch.cuda.stream(self.stream_ce):
+                            j.state.reset_moments()
+                        rebuild = True
+                ev = torch.cuda.Event()
+                ev.record(self.stream_ce)
+                ce_inflight.append((ev, ended))
+                for j in ended:
+                    active_ce.remove(j)
+                if ended or rebuild:
+                    if cycle is not None:
+                        old_cycles.append(cycle)
+                    cycle = None
+                drain_ce(lookahead)
+            elif ce_inflight:
+                drain_ce(0)
             elif nn_inflight is not None:
                 nn_inflight.synchronize()           # nothing else to run meanwhile
-            # the NeuralCE launch, once finished: exchange, counters and retirements
+            # the NeuralCE launch, once finished: moments, rebuilds, exchange and retirements
             if nn_inflight is not None and nn_inflight.query():
                 nn_inflight = None
                 with torch.cuda.stream(self.stream_nn):
                     for j in active_nn:
-                        j.after_chunk(self.nn_chunk)
-                        j.steps_done += self.nn_chunk
+                        j.after_chunk()
                         attempts += self.nn_chunk * j.state.R
-                self.stream_nn.synchronize()
                 n_ticks += 1
-                retire(active_nn, self.stream_nn)
+                for j in [j for j in active_nn if j.steps_done >= j.spec.n_steps]:
+                    self.stream_nn.synchronize()
+                    finish(j, self.stream_nn)
+                    active_nn.remove(j)
+            if old_cycles and nn_inflight is None and not ce_inflight:
+                for cy in old_cycles:
+                    cy.close()
+                old_cycles.clear()
+        torch.cuda.synchronize(self.device)
+        for cy in old_cycles + ([cycle] if cycle is not None else []):
+            cy.close()
         wall = time.perf_counter() - t0
         lat = np.array([r.latency for r in results])
         stats = dict(wall_s=wall, attempts=int(attempts), attempts_per_s=attempts / wall, n_jobs=len(results),

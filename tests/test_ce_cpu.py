@@ -38,6 +38,39 @@ def test_library_exports_every_declared_symbol():
     assert decl <= set(_lib.EXPORTED) | {"nce_last_error"}
 
 
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """every field offset and struct size of the ctypes mirrors equals the C header's (compiled with g++)."""
+    import shutil
+    import subprocess
+    from neuralce2_amd import _lib
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    pairs = [("nce_ce_model_desc", _lib.ModelDesc), ("nce_pt_job", _lib.PTJob),
+             ("nce_pt_exchange_job", _lib.ExchangeJob), ("nce_nn_model_desc", _lib.NNModelDesc),
+             ("nce_nn_job", _lib.NNJob)]
+    lines = ['#include "%s"' % os.path.join(ROOT, "include", "neuralce_hip.h"), "#include <cstdio>",
+             "#include <cstddef>", "int main() {"]
+    want = []
+    for cname, py in pairs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(ctypes_sizeof(py))
+        for f, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+            want.append(getattr(py, f).offset)
+    lines.append("return 0; }")
+    src = tmp_path / "layout.cpp"
+    src.write_text("\n".join(lines))
+    subprocess.run(["g++", "-o", str(tmp_path / "layout"), str(src)], check=True)
+    got = [int(x) for x in subprocess.run([str(tmp_path / "layout")], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert got == want
+
+
+def ctypes_sizeof(t):
+    import ctypes
+    return ctypes.sizeof(t)
+
+
 def test_orbit_order_reproduces_reference_model_energies(ptni_ce):
     """icet orbit order pinned by fits.json: ce.predict of the per-atom model (params / 256) on six orderings."""
     from neuralce2_amd.lattice import bulk_fcc
@@ -220,3 +253,18 @@ def test_cif_roundtrip(tmp_path):
     assert np.allclose(b.cell, at.cell, atol=1e-6)
     assert np.array_equal(b.numbers, at.numbers)
     assert np.allclose(b.positions, at.positions, atol=1e-6)
+
+
+def test_every_module_compiles():
+    """every Python file of the package, the oracle, the tests and the entry points is valid source."""
+    files = [os.path.join(ROOT, f) for f in ("bench.py", "__graft_entry__.py")]
+    for top in ("neuralce2_amd", "oracle", "tests"):
+        for dp, _, fs in os.walk(os.path.join(ROOT, top)):
+            files += [os.path.join(dp, f) for f in fs if f.endswith(".py")]
+    bad = []
+    for path in files:
+        try:
+            compile(open(path).read(), path, "exec")
+        except SyntaxError as e:
+            bad.append(f"{path}:{e.lineno}: {e.msg}")
+    assert not bad, bad

@@ -228,3 +228,84 @@ def test_general_kernel_injected_streams(torch_dev, ptni_ce, ptni_template, n, t
     assert np.array_equal(st.occ.cpu().numpy(), occ_ref)
     assert np.array_equal(st.n_accept.cpu().numpy(), nacc)
     assert np.max(np.abs(st.energy.cpu().numpy() - E_ref)) < (1e-9 if precision == 0 else 1e-4)
+
+
+def _pt_pair(g, sig, R, seed, sample_interval, samples_per_chunk, ladder_id=None):
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState
+    rng = np.random.default_rng(seed)
+    occ0 = np.stack([rng.permutation(sig) for _ in range(R)]).astype(np.int8)
+    betas = (1.0 / (8.617333e-5 * np.geomspace(300, 1400, R))).astype(np.float32).astype(np.float64)
+    mk = lambda: PTJobState(g, occ0, betas, ladder_id=ladder_id, seed=seed, sample_interval=sample_interval,  # noqa
+                            samples_per_chunk=samples_per_chunk)
+    return mk(), mk(), occ0, betas
+
+
+def test_device_moment_merge_matches_the_trajectory(torch_dev, ptni_ce, ptni_template):
+    """per-slot FP64 moments folded on the device after every chunk equal mean / variance of the every-step
+    trajectory by slot, across exchanges that move replicas between slots."""
+    import torch
+    g, o, sig = _setup(ptni_ce, ptni_template, 4)
+    R, chunk, n_chunks = 8, 200, 6
+    st, _, _, _ = _pt_pair(g, sig, R, 31, 1, chunk)
+    traj = []
+    for c in range(n_chunks):
+        g.ctx.pt_run_chunk([st.job(accumulate=True)], chunk, 0, 0)
+        traj.append(st.traj.cpu().numpy().copy())
+        g.ctx.pt_exchange([st.exchange_job(None, reorder_last=False)])
+    torch.cuda.synchronize()
+    e = np.concatenate(traj, 0)                              # (n_chunks * chunk, R) by slot
+    mean, var = st.moments()
+    assert np.all(st.acc_n.cpu().numpy() == n_chunks * chunk)
+    assert int(st.ex_count.item()) == n_chunks and int(st.n_swaps.item()) > 0
+    assert np.allclose(mean, e.mean(0), rtol=0, atol=1e-9 * np.abs(e).max())
+    assert np.allclose(var, e.var(0), rtol=1e-9, atol=1e-12 * np.abs(e).max() ** 2)
+
+
+def test_exchange_counter_on_the_device_equals_the_host_index(torch_dev, ptni_ce, ptni_template):
+    import torch
+    g, o, sig = _setup(ptni_ce, ptni_template, 2)
+    a, b, _, _ = _pt_pair(g, sig, 12, 5, 1, 1)
+    for k in range(1, 6):
+        g.ctx.pt_exchange([a.exchange_job(k)])
+        g.ctx.pt_exchange([b.exchange_job(None)])
+        torch.cuda.synchronize()
+        assert torch.equal(a.rep_of_slot, b.rep_of_slot) and torch.equal(a.beta_rep, b.beta_rep)
+    assert int(b.ex_count.item()) == 5
+
+
+@pytest.mark.parametrize("precision,n,ladders", [(0, 4, 1), (1, 8, 1), (1, 4, 2)])
+def test_captured_cycle_replays_equal_eager_launches(torch_dev, ptni_ce, ptni_template, precision, n, ladders):
+    """a HIP-graph cycle (2 chunks + merges + exchange) replayed three times, with eager work and synchronisation
+    in between, leaves exactly the state of the same launches issued one by one, after every replay."""
+    import torch
+    g, o, sig = _setup(ptni_ce, ptni_template, n)
+    R, chunk = 16, 300
+    lid = None if ladders == 1 else np.repeat(np.arange(ladders), R // ladders).astype(np.int32)
+    a, b, _, _ = _pt_pair(g, sig, R, 77, 100, chunk // 100, ladder_id=lid)
+    cy = g.ctx.pt_cycle([a.job(accumulate=True)], chunk, 2, precision, 0, [a.exchange_job(None)])
+    names = ("occ", "energy", "step", "n_accept", "e_min", "rep_of_slot", "slot_of_rep", "beta_rep", "acc_n",
+             "acc_mean", "acc_m2", "s1", "s2", "e_ref", "ex_count", "n_swaps", "traj")
+    for rep in range(3):
+        cy.launch(1)
+        for _ in range(2):
+            g.ctx.pt_run_chunk([b.job(accumulate=True)], chunk, precision, 0)
+        g.ctx.pt_exchange([b.exchange_job(None)])
+        scratch = torch.randn(1 << 16, device=torch_dev).sum()       # eager work between the replays
+        torch.cuda.synchronize()
+        assert torch.isfinite(scratch)
+        for nm in names:
+            assert torch.equal(getattr(a, nm), getattr(b, nm)), (rep, nm)
+    assert int(a.ex_count.item()) == 3
+    cy.close()
+
+
+def test_cycle_refuses_what_it_cannot_replay(torch_dev, ptni_ce, ptni_template):
+    import torch
+    from neuralce2_amd._lib import NCEError
+    g, o, sig = _setup(ptni_ce, ptni_template, 2)
+    a, _, _, _ = _pt_pair(g, sig, 4, 1, 10, 1)
+    with pytest.raises(NCEError, match="ex_count"):
+        g.ctx.pt_cycle([a.job()], 10, 1, 0, 0, [a.exchange_job(3)])
+    inj = tuple(torch.zeros((4, 10), dtype=t, device=torch_dev) for t in (torch.int64, torch.int64, torch.float64))
+    with pytest.raises(NCEError, match="injected"):
+        g.ctx.pt_cycle([a.job(inj=inj)], 10, 1, 0, 0, [])

@@ -76,3 +76,108 @@ def test_pt_sampler_matches_exact_enumeration_N48(torch_dev, tmp_path):
     assert np.max(np.abs(cv / C_ex - 1)) < 0.25, (cv, C_ex)
     assert abs(np.mean(cv / C_ex) - 1) < 0.08, (cv, C_ex)
     print("max |<E> - exact|", float(np.max(np.abs(E - E_ex))), "C_v ratio", np.round(cv / C_ex, 3).tolist())
+
+
+class _Preempted(Exception):
+    pass
+
+
+def _preempt_after(monkeypatch, module, n_saves):
+    """make the n-th checkpoint write of `module` raise right after the file is on disk (a killed run)."""
+    real = module._save_checkpoint
+    count = {"n": 0}
+
+    def save(*a, **k):
+        real(*a, **k)
+        count["n"] += 1
+        if count["n"] == n_saves:
+            raise _Preempted()
+    monkeypatch.setattr(module, "_save_checkpoint", save)
+
+
+def test_ce_checkpoint_resume_equals_the_uninterrupted_run(torch_dev, tmp_path, monkeypatch, ptni_ce):
+    """CE driver (reference pt_mcmc_ce.py:874-911): a run killed after its second checkpoint and started again with
+    the same settings finishes with exactly the samples, moments, rates and configurations of an uninterrupted
+    run; a checkpoint of other settings is refused."""
+    from neuralce2_amd.ce import pt_mcmc_ce as pm
+    from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
+    from tests.conftest import random_fcc
+    g = GPUClusterExpansion(ptni_ce, random_fcc(4, 0.5, 3), energy_scale=1.0)
+
+    def cfg(out, **kw):
+        base = dict(T_min=400.0, T_max=1600.0, n_replicas=8, n_steps=7000, n_equilib=1000, sample_interval=100,
+                    exchange_interval=1000, chunk_size=1000, seed=9, output_dir=str(out), checkpoint_every=2,
+                    precision="mixed", run_id="ck-test")
+        base.update(kw)
+        return pm.MCMCConfig(**base)
+    ref = pm.run_pt_mcmc(g, cfg(tmp_path / "ref", checkpoint_every=0))
+    _preempt_after(monkeypatch, pm, 2)
+    with pytest.raises(_Preempted):
+        pm.run_pt_mcmc(g, cfg(tmp_path / "ck"))
+    assert os.path.exists(tmp_path / "ck" / "checkpoint.npz")
+    with pytest.raises(RuntimeError, match="another run"):
+        pm.run_pt_mcmc(g, cfg(tmp_path / "ck", seed=10))
+    monkeypatch.undo()
+    res = pm.run_pt_mcmc(g, cfg(tmp_path / "ck"))
+    assert not os.path.exists(tmp_path / "ck" / "checkpoint.npz")
+    assert np.array_equal(res.energies, ref.energies)
+    assert np.array_equal(res.sigma_final, ref.sigma_final) and np.array_equal(res.sigma_min, ref.sigma_min)
+    assert np.array_equal(res.E_mean, ref.E_mean) and np.array_equal(res.E_var, ref.E_var)
+    assert res.swap_accept_rate == ref.swap_accept_rate and res.exchange_accept_rate == ref.exchange_accept_rate
+    assert res.E_min == ref.E_min
+
+
+def test_nce_checkpoint_resume_equals_the_uninterrupted_run(torch_dev, tmp_path, monkeypatch):
+    """NeuralCE driver (reference mcmc.py:322-399): killed after its first checkpoint (written right after a cache
+    rebuild) and resumed, the run ends with the uninterrupted run's configurations and energies."""
+    from neuralce2_amd.nce import mcmc
+    kw = dict(n_steps=4000, T_min=600.0, T_max=1200.0, n_replicas=6, exchange_every=250, chunk=500,
+              sample_interval=50, seed=4, dtype="float64", checkpoint_every=2)
+    args = (os.path.join(GOLDEN, "nipt_v3_nomax_sel.pkl"), os.path.join(GOLDEN, "init_L10_N48.cif"))
+    ref = mcmc.run_pt_mc(*args, out_dir=str(tmp_path / "ref"), **{**kw, "checkpoint_every": 0})
+    _preempt_after(monkeypatch, mcmc, 1)
+    with pytest.raises(_Preempted):
+        mcmc.run_pt_mc(*args, out_dir=str(tmp_path / "ck"), **kw)
+    assert os.path.exists(tmp_path / "ck" / "checkpoint.npz")
+    with pytest.raises(RuntimeError, match="another run"):
+        mcmc.run_pt_mc(*args, out_dir=str(tmp_path / "ck"), **{**kw, "seed": 5})
+    monkeypatch.undo()
+    res = mcmc.run_pt_mc(*args, out_dir=str(tmp_path / "ck"), **kw)
+    assert res["resumed_from_chunk"] == 2 and not os.path.exists(tmp_path / "ck" / "checkpoint.npz")
+    a = np.load(tmp_path / "ref" / "energies.npz"); b = np.load(tmp_path / "ck" / "energies.npz")
+    for k in ("E_chunk", "acc_chunk", "traj", "betas"):
+        assert a[k].shape == b[k].shape and np.allclose(a[k], b[k], rtol=0, atol=1e-12), k
+    assert np.array_equal(np.load(tmp_path / "ref" / "final_sigma.npy"), np.load(tmp_path / "ck" / "final_sigma.npy"))
+    assert res["exchange_acc"] == ref["exchange_acc"]
+
+
+@pytest.mark.parametrize("parity", [0, 1, -1])
+def test_interleaved_ladders_exchange_matches_oracle(torch_dev, ptni_ce, parity):
+    """two interleaved ladders (reference pt_mcmc_ce.py:232-260): pairs across the ladder boundary never swap, and
+    the kernel's decisions equal the oracle's ladder-masked exchange."""
+    import torch
+    from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState, _build_interleaved_ladders
+    from oracle.ce_oracle import exchange
+    from tests.conftest import random_fcc
+    g = GPUClusterExpansion(ptni_ce, random_fcc(2, 0.5, 1), energy_scale=1.0)
+    sig = g.encode(random_fcc(2, 0.5, 1)).astype(np.int8)
+    temps, betas, lid, lpos = _build_interleaved_ladders(300.0, 1400.0, 7, 2)
+    assert list(lid) == [0] * 7 + [1] * 7 and np.all(np.diff(temps[:7]) > 0)
+    R = len(temps)
+    rng = np.random.default_rng(8 + parity)
+    st = PTJobState(g, np.stack([rng.permutation(sig) for _ in range(R)]), betas.astype(np.float64), ladder_id=lid,
+                    seed=21, sample_interval=1, samples_per_chunk=1)
+    for k in range(1, 9):
+        Eset = rng.normal(0, 0.03, R)
+        st.energy.copy_(torch.tensor(Eset, device=torch_dev))
+        ros = st.rep_of_slot.cpu().numpy().copy(); brep = st.beta_rep.cpu().numpy().copy()
+        u = None if parity < 0 else rng.random(R)
+        g.ctx.pt_exchange([st.exchange_job(k, inj_parity=parity,
+                                           inj_u=None if u is None else torch.tensor(u, device=torch_dev))])
+        torch.cuda.synchronize()
+        exchange(Eset, ros, betas.astype(np.float64), brep, seed=st.seed ^ 0x9E3779B97F4A7C15, ex_index=k,
+                 ladder_id=lid, inj_parity=parity, inj_u=u)
+        got = st.rep_of_slot.cpu().numpy()
+        assert np.array_equal(got, ros) and np.array_equal(st.beta_rep.cpu().numpy(), brep)
+        assert set(got[:7]) == set(range(7)) and set(got[7:]) == set(range(7, 14))

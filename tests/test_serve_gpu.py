@@ -57,6 +57,9 @@ def test_results_same_alone_and_in_the_mix(torch_dev):
             assert np.allclose(r.E_mean, a.E_mean, rtol=1e-12, atol=0) and np.allclose(r.C_v, a.C_v, rtol=1e-6, atol=0), s.job_id
         else:
             assert np.max(np.abs(r.E_mean - a.E_mean)) < 1e-6, (r.E_mean, a.E_mean)
+        # running totals against a full recompute at retirement: FP32 swap energy changes summed in FP64 (CE),
+        # FP32 caches rebuilt every ceil(N / chunk) chunks (NeuralCE)
+        assert r.drift < (1e-8 if s.kind == "ce" else 1e-6), (s.job_id, r.drift)
         assert np.all(np.isfinite(r.C_v)) and np.all(r.C_v >= 0)
         # <E>(T) ordered with temperature on average, physical per-site energies
         assert r.E_mean[-1] >= r.E_mean[0] - 1e-3
@@ -85,3 +88,37 @@ def test_admission_respects_the_memory_budget(torch_dev):
     adm = sorted(r.admitted for r in res)
     fin = sorted(r.finished for r in res)
     assert adm[2] >= fin[0] - 1e-9                        # the third job waits for a retirement
+
+
+def _silu_factory(seed=0):
+    from neuralce2_amd.synthetic import fcc_cluster_space, fcc_cell
+    from neuralce2_amd.nce.mc_model import NNModel, silu_network
+    cs = fcc_cluster_space((9.0, 6.0, 5.0))
+    P = silu_network(len(cs), 64, 3, 1, seed=seed)
+    mu, sd = np.zeros(len(cs)), np.ones(len(cs))
+
+    def make(n):
+        return NNModel(cs, fcc_cell(n, 0.5, 0), P, mu, sd, [0.0, 3.3], act="silu", readout="sum")
+    return make
+
+
+def test_ce_ticks_overlap_a_long_neuralce_launch(torch_dev):
+    """a NeuralCE launch (3-layer SiLU, BF16) long enough to span many CE ticks on the other stream: the CE jobs
+    finish while it runs, and their results are bit-identical to running alone (regression test of the staging
+    tables and partial-sum buffers shared across streams)."""
+    from neuralce2_amd.serve import Server, JobSpec
+    specs = [JobSpec(0, n=6, x=0.5, n_temps=8, n_steps=1500, seed=40, kind="nce"),
+             JobSpec(1, n=4, x=0.5, n_temps=8, n_steps=6000, seed=41),
+             JobSpec(2, n=5, x=0.3, n_temps=12, n_steps=6000, seed=42, n_equilib=1500),
+             JobSpec(3, n=3, x=0.6, n_temps=6, n_steps=4500, seed=43)]
+    srv = Server(chunk=300, nn_chunk=1500, keep_occ=True, nn_precision="bf16", nce_model=_silu_factory())
+    mix, _ = srv.run(specs)
+    nce = mix[0]
+    for s, r in zip(specs[1:], mix[1:]):
+        assert r.finished < nce.finished, (s.job_id, r.finished, nce.finished)
+        alone, _ = srv.run([s])
+        a = alone[0]
+        assert np.array_equal(r.occ, a.occ), s.job_id
+        assert np.array_equal(r.E_mean, a.E_mean) and np.array_equal(r.C_v, a.C_v), s.job_id
+        assert r.drift < 1e-8
+    assert nce.drift < 1e-5 and np.all(np.isfinite(nce.E_mean))
