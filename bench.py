@@ -173,6 +173,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=100.0, help="requests per second (Poisson)")
     ap.add_argument("--job-steps", type=int, default=100000)
     ap.add_argument("--n-max", type=int, default=32)
+    ap.add_argument("--dump-moments", default=None,
+                    help="write the per-temperature <E> and C_v of the timed run (npz) for precision comparisons")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of 1 GiB (the known-size kernel that scales rocprofv3's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes)")
@@ -324,6 +326,15 @@ def main():
     k_ms = np.array([a.elapsed_time(b) for a, b in evs])
     attempts = R * chunk * args.steps
     value = attempts / dt
+    # outside the timed region: running totals against an FP64 full recompute, and the per-slot moments
+    full = torch.empty_like(st.energy)
+    g.ctx.ce_energy(g.model, st.occ, full)
+    drift = float((full - st.energy).abs().max().item()) / len(at)
+    if args.dump_moments:
+        mean, var = st.moments()
+        T64 = temps.astype(np.float64)
+        np.savez(args.dump_moments, temps=T64, E_mean=mean / len(at), C_v=var / (KB * T64) ** 2 / len(at),
+                 steps=chunk * (args.warmup + args.steps), precision=cfg["precision"], drift=drift)
     bytes_launch = R * chunk * B_ALG_PER_ATTEMPT
     achieved = bytes_launch / (k_ms.mean() * 1e-3) / 1e9
     line = {
@@ -352,6 +363,7 @@ def main():
                      "bytes_per_attempt_alg": B_ALG_PER_ATTEMPT},
         "cpu_baseline": None,
         "acceptance": float(st.n_accept.sum().item()) / (R * chunk * (args.warmup + args.steps)),
+        "drift_vs_full_recompute": {"ev_per_site": drift, "steps_per_replica": chunk * (args.warmup + args.steps)},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(g.info, np.broadcast_to(sig, (R, len(sig))), betas,

@@ -64,6 +64,10 @@ class GPUClusterExpansion:
         self.device = torch.device("cuda", device)
         self.ctx = Context.get(device)
         self.model = self.ctx.ce_model_create(info)
+        # rounding scale of one FP32 swap energy change: unit roundoff x factor x the largest sum of |coefficient|
+        # over a window, for the two windows of a swap (the re-anchor schedule of the mixed form is set from it)
+        coef = np.abs(np.asarray(info['template'].ent_coef, np.float64))
+        self.fp32_step_error = 2.0 ** -24 * abs(float(info['factor'])) * 2.0 * float(coef.sum(axis=-1).max())
         self.spin_tables = None
         self._has_spin = False
 

@@ -97,6 +97,8 @@ class _CEJob:
         self.steps_done = 0
         self.equilibrating = spec.n_equilib > 0
         self.drift = 0.0
+        self.anchor_every = 0                         # set by the server from the precision
+        self.n_ticks = 0
 
     @staticmethod
     def bytes_needed(spec):
@@ -113,7 +115,7 @@ class _CEJob:
         acc = float(st.n_accept.sum().item()) / max(1, self.steps_done * st.R)
         full = torch.empty_like(st.energy)
         st.gce.ctx.ce_energy(st.gce.model, st.occ, full)
-        self.drift = float((full - st.energy).abs().max().item()) / self.N
+        self.drift = max(float((full - st.energy).abs().max().item()), float(st.drift.item())) / self.N
         occ = None
         if keep_occ:
             rep = st.rep_of_slot.cpu().numpy()
@@ -182,31 +184,35 @@ class _NCEJob:
         return mean, cv, acc, occ
 
 
-def autotune_tile(ctx, states, precision, candidates=(4, 8, 16), n_steps=256):
+def autotune_tile(ctx, states, precision, candidates=(4, 8, 16), n_steps=256, stream=None):
     """Pick the speculative kernel's tile (steps of a replica evaluated in parallel) for this set of CE jobs by timing
-    one short chunk per candidate on copies of the jobs' state (the jobs themselves do not move)."""
+    one short chunk per candidate on copies of the jobs' state (the jobs themselves do not move).  Everything runs
+    and is timed on `stream` (the CE jobs' stream), so launches on other streams are not waited for."""
     import torch
     from .ce.pt_mcmc_ce import PTJobState
-    torch.cuda.synchronize()                                      # the states may still be written on their stream
-    copies = []
-    for st in states:
-        c = PTJobState.__new__(PTJobState)
-        c.__dict__.update(st.__dict__)
-        for name in ("occ", "energy", "beta_rep", "slot_of_rep", "step", "n_accept", "e_min", "e_ref", "s1", "s2"):
-            setattr(c, name, getattr(st, name).clone())
-        c.traj = None
-        copies.append(c)
-    best, best_t = candidates[0], float("inf")
-    for tile in candidates:
-        jobs = [c.job(accumulate=False) for c in copies]
-        ctx.pt_run_chunk(jobs, n_steps, precision, tile)          # warm-up (code objects, caches)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        ctx.pt_run_chunk(jobs, n_steps, precision, tile)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t
-        if dt < best_t:
-            best, best_t = tile, dt
+    stream = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(stream):
+        copies = []
+        for st in states:
+            c = PTJobState.__new__(PTJobState)
+            c.__dict__.update(st.__dict__)
+            for name in ("occ", "energy", "beta_rep", "slot_of_rep", "step", "n_accept", "e_min", "e_ref", "s1",
+                         "s2"):
+                setattr(c, name, getattr(st, name).clone())
+            c.traj = None
+            copies.append(c)
+        best, best_t = candidates[0], float("inf")
+        for tile in candidates:
+            jobs = [c.job(accumulate=False) for c in copies]
+            ctx.pt_run_chunk(jobs, n_steps, precision, tile, stream)        # warm-up (code objects, caches)
+            t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0.record(stream)
+            ctx.pt_run_chunk(jobs, n_steps, precision, tile, stream)
+            t1.record(stream)
+            t1.synchronize()
+            dt = t0.elapsed_time(t1)
+            if dt < best_t:
+                best, best_t = tile, dt
     return best
 
 
@@ -269,13 +275,15 @@ class Server:
         key = (min(4, (n_items + n_cu - 1) // n_cu), self.precision,
                int(math.log2(max(1, min(j.N for j in ce_jobs)))))
         if key not in self._tile_cache:
-            self._tile_cache[key] = autotune_tile(self.ctx, [j.state for j in ce_jobs], self.precision)
+            self._tile_cache[key] = autotune_tile(self.ctx, [j.state for j in ce_jobs], self.precision,
+                                                  stream=self.stream_ce)
         return self._tile_cache[key]
 
     def run(self, specs: List[JobSpec], log=None):
         """Serve the request stream; returns (results sorted by job id, stats)."""
         import torch
         from collections import deque
+        from .ce.pt_mcmc_ce import anchor_interval_chunks
         pending = sorted(specs, key=lambda s: (s.arrival, s.job_id))
         for s in pending:
             ch = self.chunk if s.kind == "ce" else self.nn_chunk
@@ -331,6 +339,9 @@ class Server:
                            else _NCEJob(spec, self._nn(spec.n), self.nn_chunk, self.nn_precision))
                 job.admitted = time.perf_counter() - t0
                 job.mem = self._bytes(spec)
+                if spec.kind == "ce":
+                    job.anchor_every = anchor_interval_chunks(job.N, self.chunk, self.precision,
+                                                              job.state.gce.fp32_step_error)
                 mem_used += job.mem
                 if spec.kind == "ce":
                     active_ce.append(job)
@@ -362,7 +373,11 @@ class Server:
                 ended, rebuild = [], False
                 for j in active_ce:
                     j.steps_done += self.chunk
+                    j.n_ticks += 1
                     attempts += self.chunk * j.state.R
+                    if j.anchor_every and j.n_ticks % j.anchor_every == 0 and j.steps_done < j.spec.n_steps:
+                        with torch.cuda.stream(self.stream_ce):
+                            j.state.anchor()            # FP64 re-anchor between two replays, stream-ordered
                     if j.steps_done >= j.spec.n_steps:
                         ended.append(j)
                     elif j.equilibrating and j.steps_done >= j.spec.n_equilib:

@@ -252,3 +252,66 @@ int oracle_pt_exchange(int32_t R, const double *E, int32_t *rep_of_slot, const d
     *n_swaps += ns;
     return 0;
 }
+
+/* ── exact enumeration ────────────────────────────────────────────────────────────────────────────────────────
+ * Energies of every configuration of n binary sites with k of species 1, in lexicographic order of the k-subsets:
+ * E = e0 + sum_t coef[t] prod_{m < mo, site[t][m] >= 0} s[site[t][m]], s = +1 for species 1 and -1 for species 0.
+ * The sequence is cut into ranges, one per thread; each range starts from its subset unranked from the
+ * combinatorial number system and steps with the next-subset rule.  n <= 62. */
+static double binom(int n, int k) {
+    if (k < 0 || k > n) return 0.0;
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
+    return r;
+}
+
+static void unrank_subset(int n, int k, int64_t rank, int *c) {
+    int x = 0;
+    for (int i = 0; i < k; ++i) {
+        for (;; ++x) {
+            const int64_t cnt = (int64_t)llround(binom(n - x - 1, k - i - 1));    /* subsets starting with x */
+            if (rank < cnt) break;
+            rank -= cnt;
+        }
+        c[i] = x++;
+    }
+}
+
+int oracle_enumerate_energies(int32_t n, int32_t k, int64_t n_terms, int32_t mo, const int32_t *site,
+                              const double *coef, double e0, double *E_out) {
+    if (n < 1 || n > 62 || k < 0 || k > n || mo < 1) return -1;
+    const int64_t total = (int64_t)llround(binom(n, k));
+    int n_threads = 1;
+#ifdef _OPENMP
+    n_threads = omp_get_max_threads();
+#endif
+#pragma omp parallel for schedule(static, 1)
+    for (int t = 0; t < n_threads; ++t) {
+        const int64_t lo = total * t / n_threads, hi = total * (t + 1) / n_threads;
+        int c[64];
+        double s[64];
+        if (lo < hi) {
+            unrank_subset(n, k, lo, c);
+            for (int64_t r = lo; r < hi; ++r) {
+                for (int i = 0; i < n; ++i) s[i] = -1.0;
+                for (int i = 0; i < k; ++i) s[c[i]] = 1.0;
+                double e = e0;
+                for (int64_t q = 0; q < n_terms; ++q) {
+                    double p = coef[q];
+                    for (int m = 0; m < mo; ++m) {
+                        const int st = site[q * mo + m];
+                        if (st >= 0) p *= s[st];
+                    }
+                    e += p;
+                }
+                E_out[r] = e;
+                int i = k - 1;                                   /* next k-subset in lexicographic order */
+                while (i >= 0 && c[i] == n - k + i) --i;
+                if (i < 0) break;
+                ++c[i];
+                for (int j = i + 1; j < k; ++j) c[j] = c[j - 1] + 1;
+            }
+        }
+    }
+    return 0;
+}

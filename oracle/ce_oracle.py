@@ -55,6 +55,8 @@ def _lib():
     lib.oracle_philox4x32_10.restype = None
     lib.oracle_draw.argtypes = [_p, C.c_uint64, C.c_uint64, C.c_uint32, _p, _p, _p]
     lib.oracle_draw.restype = None
+    lib.oracle_enumerate_energies.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int32, _p, _p, C.c_double, _p]
+    lib.oracle_enumerate_energies.restype = C.c_int
     return lib
 
 
@@ -163,13 +165,14 @@ def exchange(E, rep_of_slot, beta_slot, beta_rep, seed=0, ex_index=0, ladder_id=
 
 
 # ── independent brute force ──────────────────────────────────────────────────────────────────────────
-def brute_energy(cs, parameters, atoms, energy_scale=1.0):
-    """zerolet + sum_orbit eci_orbit * scale * <prod of spins over the orbit's clusters in the supercell>.
+def brute_terms(cs, parameters, atoms, energy_scale=1.0):
+    """Every cluster of the supercell as an explicit term: sites (n_terms, max_order) with -1 padding, coefficient
+    eci_orbit * scale / (n_instances * n_cells), and the constant zerolet * scale, so that
+    E = e0 + sum_t coef_t prod_{m} s[site_tm] with binary spins s = -1 (species 0) / +1 (species 1).
 
     Clusters are placed by Cartesian position: for every primitive cell (lattice translation inside the supercell) and
     every cluster instance of the orbit, each member's position is wrapped into the supercell and matched to the
-    site at that position.  Binary spins: species index 0 -> -1, 1 -> +1."""
-    from neuralce2_amd.lattice import ATOMIC_NUMBERS
+    site at that position."""
     prim = cs.primitive_structure
     P = prim.cell; S = atoms.cell
     fb = prim.get_scaled_positions()
@@ -185,10 +188,6 @@ def brute_energy(cs, parameters, atoms, energy_scale=1.0):
     trans = grid[inside]
     assert len(trans) == n_cells, (len(trans), n_cells)
     fs = atoms.get_scaled_positions()
-    spins = np.zeros(len(atoms))
-    for i, z in enumerate(atoms.numbers):
-        spec = sorted(cs.chemical_symbols[0], key=lambda x: ATOMIC_NUMBERS[x])
-        spins[i] = -1.0 if spec.index({v: k for k, v in ATOMIC_NUMBERS.items()}[int(z)]) == 0 else 1.0
 
     def site_at(xyz):
         f = np.mod(xyz @ Sinv, 1.0)
@@ -198,14 +197,41 @@ def brute_energy(cs, parameters, atoms, energy_scale=1.0):
         j = err.argmin(1)
         assert np.all(err[np.arange(len(j)), j] < 1e-6)
         return j
-    E = float(parameters[0]) * energy_scale
+    mo = max(o.order for o in cs.orbits)
+    sites, coef = [], []
     for oi, o in enumerate(cs.orbits):
-        tot = 0.0
+        c = float(parameters[oi + 1]) * energy_scale / (len(o.clusters) * n_cells)
         for inst in o.clusters:
-            prod = np.ones(len(trans))
-            for b, n in inst:
-                xyz = (fb[b] + np.asarray(n) + trans) @ P
-                prod *= spins[site_at(xyz)]
-            tot += prod.sum()
-        E += float(parameters[oi + 1]) * energy_scale * tot / (len(o.clusters) * n_cells)
-    return E
+            cols = [site_at((fb[b] + np.asarray(n) + trans) @ P) for b, n in inst]
+            blk = np.full((len(trans), mo), -1, np.int32)
+            blk[:, :len(cols)] = np.stack(cols, 1)
+            sites.append(blk)
+            coef.append(np.full(len(trans), c))
+    return np.concatenate(sites), np.concatenate(coef), float(parameters[0]) * energy_scale
+
+
+def spins_of(cs, atoms):
+    from neuralce2_amd.lattice import ATOMIC_NUMBERS, CHEMICAL_SYMBOLS
+    spec = sorted(cs.chemical_symbols[0], key=lambda x: ATOMIC_NUMBERS[x])
+    return np.array([-1.0 if spec.index(CHEMICAL_SYMBOLS[int(z)]) == 0 else 1.0 for z in atoms.numbers])
+
+
+def brute_energy(cs, parameters, atoms, energy_scale=1.0):
+    """zerolet + sum_orbit eci_orbit * scale * <prod of spins over the orbit's clusters in the supercell>, from the
+    explicit cluster list of brute_terms.  Binary spins: species index 0 -> -1, 1 -> +1."""
+    sites, coef, e0 = brute_terms(cs, parameters, atoms, energy_scale)
+    s = np.append(spins_of(cs, atoms), 1.0)            # index -1 (padding) -> +1
+    return e0 + float((coef * np.prod(s[sites], axis=1)).sum())
+
+
+def enumerate_energies(sites, coef, e0, n, k):
+    """energies of all C(n, k) configurations with k sites of species 1 (lexicographic order of the k-subsets),
+    from the explicit cluster list; OpenMP over ranges of the sequence."""
+    from math import comb
+    total = comb(n, k)
+    out = np.empty(total, np.float64)
+    sites = np.ascontiguousarray(sites, np.int32); coef = np.ascontiguousarray(coef, np.float64)
+    rc = lib().oracle_enumerate_energies(int(n), int(k), len(coef), sites.shape[1], sites.ctypes.data,
+                                         coef.ctypes.data, float(e0), out.ctypes.data)
+    assert rc == 0, rc
+    return out

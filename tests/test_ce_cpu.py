@@ -268,3 +268,39 @@ def test_every_module_compiles():
         except SyntaxError as e:
             bad.append(f"{path}:{e.lineno}: {e.msg}")
     assert not bad, bad
+
+
+def _exact_cell():
+    """2 x 3 x 4 primitive fcc supercell (24 sites; shortest image 5.55 A), pairs <= 5 A and triplets <= 4 A, so no
+    cluster holds a site twice; per-site ECIs N(0, 20 meV) from a seed."""
+    from neuralce2_amd.clusterspace import ClusterSpace
+    from neuralce2_amd.lattice import bulk_fcc
+    cs = ClusterSpace(bulk_fcc("Ni", 3.924, cubic=False), [5.0, 4.0], [["Ni", "Pt"]])
+    at = bulk_fcc("Ni", 3.924, cubic=False).repeat((2, 3, 4))
+    params = np.zeros(len(cs)); params[1:] = np.random.default_rng(0).normal(0, 0.02, len(cs) - 1)
+    return cs, at, params
+
+
+def test_exact_enumeration_energies_match_brute_force():
+    """the enumerator's energies of C(24, 12) configurations (lexicographic k-subsets) equal the brute-force energy
+    of the same configurations, and every energy is invariant under the lattice's translations."""
+    from itertools import combinations, islice
+    from math import comb
+    from oracle.ce_oracle import brute_terms, brute_energy, enumerate_energies
+    cs, at, params = _exact_cell()
+    N = len(at)
+    sites, coef, e0 = brute_terms(cs, params, at, float(N))
+    E = enumerate_energies(sites, coef, e0, N, 12)
+    assert len(E) == comb(24, 12)
+    picks = [0, 1, 17, 5000, 123456, len(E) // 2, len(E) - 2, len(E) - 1]
+    subsets = {}
+    for r, c in enumerate(combinations(range(N), 12)):
+        if r in picks:
+            subsets[r] = c
+        if r > max(picks):
+            break
+    for r, c in subsets.items():
+        a = at.copy(); Z = np.full(N, 28); Z[list(c)] = 78; a.numbers = Z
+        assert abs(E[r] - brute_energy(cs, params, a, float(N))) < 1e-12, r
+    # the energies take few distinct values per symmetry class: the minimum is reached at least N / |orbit| times
+    assert np.sum(np.isclose(E, E.min(), rtol=0, atol=1e-12)) >= 2

@@ -309,3 +309,48 @@ def test_cycle_refuses_what_it_cannot_replay(torch_dev, ptni_ce, ptni_template):
     inj = tuple(torch.zeros((4, 10), dtype=t, device=torch_dev) for t in (torch.int64, torch.int64, torch.float64))
     with pytest.raises(NCEError, match="injected"):
         g.ctx.pt_cycle([a.job(inj=inj)], 10, 1, 0, 0, [])
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_pt_statistics_match_exact_enumeration(torch_dev, precision):
+    """The CE parallel-tempering sampler against the exact thermodynamics of a 24-site cell (every one of the
+    C(24, 12) configurations enumerated by the oracle): <E>(T) within 3e-4 eV/site and C_v(T) within 6 % at 12
+    temperatures 250-2000 K, from 3 x 10^5 steps per replica (the first tenth dropped; a few standard errors)."""
+    from neuralce2_amd.ce_io import ClusterExpansion
+    from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
+    from neuralce2_amd.ce.pt_mcmc_ce import PTJobState, anchor_interval_chunks
+    from oracle.ce_oracle import brute_terms, enumerate_energies
+    from tests.test_ce_cpu import _exact_cell
+    cs, at, params = _exact_cell()
+    N = len(at)
+    sites, coef, e0 = brute_terms(cs, params, at, float(N))
+    E_all = enumerate_energies(sites, coef, e0, N, 12)
+    temps = np.geomspace(250.0, 2000.0, 12).astype(np.float32)
+    T64 = temps.astype(np.float64)
+    ex_E, ex_C = [], []
+    for T in T64:
+        b = 1.0 / (8.617333e-5 * T)
+        w = np.exp(-b * (E_all - E_all.min()))
+        m1 = (w * E_all).sum() / w.sum(); m2 = (w * E_all ** 2).sum() / w.sum()
+        ex_E.append(m1 / N); ex_C.append((m2 - m1 * m1) * b * b / N)
+    ex_E, ex_C = np.array(ex_E), np.array(ex_C)
+    Z = np.full(N, 28); Z[np.random.default_rng(4).choice(N, 12, replace=False)] = 78; at.numbers = Z
+    g = GPUClusterExpansion(ClusterExpansion(cs, params), at, energy_scale=float(N))
+    sig = g.encode(at).astype(np.int8)
+    betas = (1.0 / (8.617333e-5 * T64)).astype(np.float32).astype(np.float64)
+    st = PTJobState(g, sig, betas, seed=31, sample_interval=1000, samples_per_chunk=1)
+    chunk, n_chunks = 1000, 300
+    every = anchor_interval_chunks(N, chunk, precision, g.fp32_step_error)
+    for c in range(n_chunks):
+        g.ctx.pt_run_chunk([st.job(accumulate=c >= n_chunks // 10)], chunk, precision, 0)
+        g.ctx.pt_exchange([st.exchange_job(None)])
+        if every and (c + 1) % every == 0:
+            st.anchor()
+        if c + 1 == n_chunks // 10:
+            st.reset_moments()
+    mean, var = st.moments()
+    E_mc = mean / N; C_mc = var / (8.617333e-5 * T64) ** 2 / N
+    dE = np.abs(E_mc - ex_E); dC = np.abs(C_mc / ex_C - 1)
+    print(f"precision {precision}: max |<E> - exact| {dE.max():.2e} eV/site, C_v ratio "
+          f"{np.round(C_mc / ex_C, 3).tolist()}")
+    assert dE.max() < 3e-4 and dC.max() < 0.06, (dE, C_mc / ex_C)

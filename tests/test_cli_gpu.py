@@ -53,10 +53,12 @@ def test_cli_nce_pt_with_init_npz(torch_dev, tmp_path):
     assert z["traj"].shape[0] == 4 and np.all(z["traj"] > -6.1) and np.all(z["traj"] < -5.8)
 
 
-def test_pt_sampler_matches_exact_enumeration_N48(torch_dev, tmp_path):
+@pytest.mark.parametrize("dtype", ["float64", "bfloat16"])
+def test_pt_sampler_matches_exact_enumeration_N48(torch_dev, tmp_path, dtype):
     """<E>(T) within 3e-3 eV/atom and C_v(T) within 25 % of the exact values at every temperature, and within 8 %
     on average over the ladder, from 6e5 steps per replica, the first quarter dropped (statistical bounds of a few
-    standard errors; in the ordered phase below ~650 K the rare antisite excitations need far longer runs)."""
+    standard errors; in the ordered phase below ~650 K the rare antisite excitations need far longer runs); in FP64
+    and with the graph-convolution projections on BF16 MFMA."""
     from neuralce2_amd.nce.mcmc import run_pt_mc
     from neuralce2_amd.analysis import heat_capacity
     ex = json.load(open(os.path.join(GOLDEN, "exact_N48.json")))
@@ -65,17 +67,17 @@ def test_pt_sampler_matches_exact_enumeration_N48(torch_dev, tmp_path):
     out = tmp_path / "exact"
     run_pt_mc(os.path.join(GOLDEN, "nipt_v3_nomax_sel.pkl"), os.path.join(GOLDEN, "init_L10_N48.cif"),
               n_steps=600000, T_grid=temps, exchange_every=100, chunk=2000, sample_interval=20, seed=5,
-              out_dir=str(out), dtype="float64")
+              out_dir=str(out), dtype=dtype)
     z = np.load(out / "energies.npz", allow_pickle=False)
     e = z["traj"][:, z["traj"].shape[1] // 4:].T               # drop the first quarter (equilibration)
     E = e.mean(0)
     cv = heat_capacity(e, z["temps"], 48)
     E_ex = np.interp(temps, Tx, Ex)
     C_ex = np.interp(temps, Tx, Cx)
+    print(dtype, "max |<E> - exact|", float(np.max(np.abs(E - E_ex))), "C_v ratio", np.round(cv / C_ex, 3).tolist())
     assert np.max(np.abs(E - E_ex)) < 3e-3, (E - E_ex)
     assert np.max(np.abs(cv / C_ex - 1)) < 0.25, (cv, C_ex)
     assert abs(np.mean(cv / C_ex) - 1) < 0.08, (cv, C_ex)
-    print("max |<E> - exact|", float(np.max(np.abs(E - E_ex))), "C_v ratio", np.round(cv / C_ex, 3).tolist())
 
 
 class _Preempted(Exception):

