@@ -376,3 +376,73 @@ def test_lds_placements_agree(torch_dev, ptni_nce, monkeypatch, cell):
             assert np.array_equal(res[cap][0], ref[0]), (prec, cap)
             assert np.array_equal(res[cap][1], ref[1]), (prec, cap)
             assert np.max(np.abs(res[cap][2] - ref[2])) < tol, (prec, cap)
+
+
+def test_silu_network_16cubed_chain_matches_oracle(torch_dev):
+    """The 16,384-site NeuralCE workload (fcc 16^3 cubic, x = 0.5, 3 SiLU layers of width 64 over the 12 nearest
+    neighbours, 41 layer-0 channels, summed readout), 8 replicas, an injected chain of 8 steps: half of the
+    proposals pair sites half a cell apart (their balls share no site: the fixed-table path), half are uniform.
+    FP64: occupations bit-exact with the FP64 oracle's replay, running totals within 1e-9 eV.  FP32: the same
+    decisions, totals within 1e-3 eV of the oracle energy of the final configuration (16,384 sites, |E| ~ 1.7e3 eV).
+    BF16 projections: totals within 0.5 eV of the oracle energy of their own final configuration.  First-step energy
+    changes without commit (nce_nn_delta): FP64 1e-9 eV, FP32 1e-5 relative, BF16 3 % + 1e-3 eV."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from neuralce2_amd.synthetic import fcc_cluster_space, fcc_cell
+    from neuralce2_amd.nce.mc_model import NNModel, silu_network
+    from neuralce2_amd.nce.mcmc import NNJobState
+    from oracle.nce_oracle import Oracle, _site_lookup
+    cs = fcc_cluster_space((9.0, 6.0, 5.0))
+    P = silu_network(len(cs), 64, 3, 1, seed=0)
+    mu, sd = np.zeros(len(cs)), np.ones(len(cs))
+    at = fcc_cell(16, 0.5, 0)
+    nn = NNModel(cs, at, P, mu, sd, [0.0, 3.3], act="silu", readout="sum")
+    o = Oracle(cs, at, [0.0, 3.3], P, mu, sd, kind="silu", readout="sum")
+    perm, inv, N = nn.geo["perm"], nn.geo["inv_perm"], nn.N
+    R, steps = 8, 8
+    rng = np.random.default_rng(12)
+    sig_o = (at.numbers == 78).astype(np.int8)
+    occ_o = np.stack([rng.permutation(sig_o) for _ in range(R)])
+    ia = rng.integers(0, N, (R, steps))
+    far = _site_lookup(at)(at.positions[ia.ravel()] + at.cell.sum(0) / 2).reshape(R, steps)
+    ib = np.where(np.arange(steps)[None, :] % 2 == 0, far, rng.integers(0, N, (R, steps)))
+    u = rng.random((R, steps))
+    betas = (1.0 / (8.617333262e-5 * np.geomspace(600, 2400, R))).astype(np.float32).astype(np.float64)
+    inj = tuple(torch.tensor(x, device=torch_dev) for x in (inv[ia], inv[ib], u))
+    runs = {}
+    for prec in (0, 1, 2):
+        st = NNJobState(nn, occ_o[:, perm], betas, precision=prec, sample_interval=1, samples_per_chunk=steps)
+        first = tuple(torch.tensor(x[:, :1].copy(), device=torch_dev) for x in (inv[ia], inv[ib], u))
+        dE = torch.empty(R, dtype=torch.float64, device=torch_dev)
+        nn.ctx.nn_delta([st.job(first)], dE, prec)
+        st.run(steps, inj)
+        torch.cuda.synchronize()
+        runs[prec] = (st.occ.cpu().numpy()[:, inv], st.energy.cpu().numpy(), dE.cpu().numpy())
+
+    def replay(r):
+        occ = occ_o[r:r + 1].copy()
+        E0 = np.array([o.energy(occ[0])])
+        s1 = occ[0].copy(); s1[ia[r, 0]], s1[ib[r, 0]] = occ[0, ib[r, 0]], occ[0, ia[r, 0]]
+        d1 = o.energy(s1) - E0[0]
+        E = E0.copy()
+        o.run_chunk(occ, E, betas[r:r + 1], (ia[r:r + 1], ib[r:r + 1], u[r:r + 1]), 1.0)
+        return occ[0], E[0], d1
+    with ThreadPoolExecutor(8) as ex:
+        ref = list(ex.map(replay, range(R)))
+    occ_ref = np.stack([x[0] for x in ref]); E_ref = np.array([x[1] for x in ref]); d_ref = np.array([x[2] for x in ref])
+    occ64, E64, d64 = runs[0]
+    assert np.array_equal(occ64, occ_ref)
+    assert np.max(np.abs(E64 - E_ref)) < 1e-9, np.abs(E64 - E_ref)
+    assert np.max(np.abs(d64 - d_ref)) < 1e-9
+    occ32, E32, d32 = runs[1]
+    assert np.array_equal(occ32, occ_ref)
+    assert np.max(np.abs(E32 - E_ref)) < 1e-3, np.abs(E32 - E_ref)
+    assert np.all(np.abs(d32 - d_ref) <= 1e-5 * np.abs(d_ref) + 1e-6)
+    occ16, E16, d16 = runs[2]
+    with ThreadPoolExecutor(8) as ex:
+        E16_ref = np.array(list(ex.map(o.energy, occ16)))
+    print(f"16^3 SiLU: FP32 total error {np.abs(E32 - E_ref).max():.2e} eV, BF16 {np.abs(E16 - E16_ref).max():.2e} eV; "
+          f"BF16 first-step dE relative error {np.max(np.abs(d16 - d_ref) / np.abs(d_ref)):.2e}; "
+          f"{int(np.sum(np.all(occ16 == occ_ref, axis=1)))}/{R} BF16 chains on the FP64 path")
+    assert np.max(np.abs(E16 - E16_ref)) < 0.5
+    assert np.all(np.abs(d16 - d_ref) <= 0.03 * np.abs(d_ref) + 1e-3)
