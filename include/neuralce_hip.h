@@ -131,6 +131,16 @@ int nce_ce_model_create(nce_ctx *ctx, const nce_ce_model_desc *desc, int32_t *mo
 int nce_ce_energy(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_replicas, int64_t n_sites,
                   double *energy, void *stream);
 
+/* nce_ce_energy without any allocation: no reference counterpart; for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The per-block partial sums live
+ * in a fixed per-GPU array of NCE_NOALLOC_PARTIALS doubles (8 MiB, held until the process ends); a call needs
+ * n_replicas * ceil(n_cells / 256) of them and fails with NCE_ERR_ARG beyond that.  Every call on a GPU uses that
+ * GPU's one array, so calls that overlap on one GPU (other streams, other contexts, other threads) race; results
+ * are those of nce_ce_energy. */
+#define NCE_NOALLOC_PARTIALS (1 << 20)
+int nce_ce_energy_noalloc(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_replicas, int64_t n_sites,
+                          double *energy, void *stream);
+
 /* Swap energy change of explicit proposals (one per replica): dE[r] for swapping ia[r] and ib[r] of occ[r].
  * Replaces delta_energy_a (neuralce/ce/ce_gpu_implicit.py:181-191). */
 int nce_ce_delta(nce_ctx *ctx, int32_t model, const int8_t *occ, int32_t n_replicas, int64_t n_sites,

@@ -72,6 +72,7 @@ def _load():
     lib.nce_ctx_destroy.restype = None
     lib.nce_ce_model_create.argtypes = [_p, C.POINTER(ModelDesc), C.POINTER(C.c_int32)]
     lib.nce_ce_energy.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p]
+    lib.nce_ce_energy_noalloc.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p]
     lib.nce_ce_delta.argtypes = [_p, C.c_int32, _p, C.c_int32, C.c_int64, _p, _p, _p, C.c_int32, _p]
     lib.nce_pt_run_chunk.argtypes = [_p, _p, C.c_int32, C.c_int64, C.c_int32, C.c_int32, _p]
     lib.nce_pt_exchange.argtypes = [_p, _p, C.c_int32, _p]
@@ -93,6 +94,7 @@ def _load():
 lib = _load()
 
 EXPORTED = ("nce_last_error", "nce_ctx_create", "nce_ctx_destroy", "nce_ce_model_create", "nce_ce_energy",
+            "nce_ce_energy_noalloc",
             "nce_ce_delta", "nce_pt_run_chunk", "nce_pt_exchange", "nce_pt_cycle_create", "nce_pt_cycle_launch",
             "nce_pt_cycle_destroy", "nce_philox4x32_10", "nce_nn_model_create", "nce_nn_sizes", "nce_nn_full_cache",
             "nce_nn_run_chunk", "nce_nn_delta")
@@ -188,6 +190,11 @@ class Context:
     def ce_energy(self, model, occ, energy, stream=None):
         R, N = occ.shape
         check(lib.nce_ce_energy(self.h, model, ptr(occ), R, N, ptr(energy), stream_ptr(stream)))
+
+    def ce_energy_noalloc(self, model, occ, energy, stream=None):
+        """nce_ce_energy with the partial sums in the library's fixed per-GPU scratch (no allocation)."""
+        R, N = occ.shape
+        check(lib.nce_ce_energy_noalloc(self.h, model, ptr(occ), R, N, ptr(energy), stream_ptr(stream)))
 
     def ce_delta(self, model, occ, ia, ib, dE, precision=PREC_FP64, stream=None):
         R, N = occ.shape

@@ -153,8 +153,14 @@ struct NNModelHost {
     NNModelDev m;
 };
 
+// Partial sums of nce_ce_energy_noalloc: file-scope device memory, one copy per GPU that loads the library, held
+// until the process ends (8 MiB).  In host code `scratch` only names the symbol: its device address comes from
+// hipGetSymbolAddress at context creation (kept in nce_ctx::noalloc_partial) and is what kernels receive.
+static __device__ __align__(256) double scratch[NCE_NOALLOC_PARTIALS];
+
 struct nce_ctx {
     int device = 0;
+    double *noalloc_partial = nullptr;   // this device's address of `scratch`
     int n_cu = 0;
     size_t max_lds = 0;
     std::vector<ModelDev> models;
@@ -1354,8 +1360,11 @@ extern "C" int nce_ctx_create(int32_t device_ordinal, nce_ctx **out) {
     HIP_TRY(hipSetDevice(device_ordinal));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device_ordinal));
+    double *noalloc = nullptr;                    // with this device current: its copy of the no-allocation scratch
+    HIP_TRY(hipGetSymbolAddress((void **)&noalloc, HIP_SYMBOL(scratch)));
     nce_ctx *c = new nce_ctx();
     c->device = device_ordinal;
+    c->noalloc_partial = noalloc;
     c->n_cu = prop.multiProcessorCount;
     c->max_lds = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
     *out = c;
@@ -1628,29 +1637,54 @@ static int allow_lds(K kernel, size_t lds) {
     return NCE_OK;
 }
 
-extern "C" int nce_ce_energy(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, double *E,
-                             void *stream) {
+// full energies: argument checks and the number of per-block partial sums per replica
+static int ce_energy_blocks(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, const double *E,
+                            int *nblk) {
     int rc = check_model(c, model);
     if (rc) return rc;
     const ModelDev &m = c->models[model];
     if (N != m.n_sites) return set_err(NCE_ERR_ARG, "n_sites does not match the model's cell");
     if (R < 1 || !occ || !E) return set_err(NCE_ERR_ARG, "bad replica count or NULL buffer");
-    HIP_TRY(hipSetDevice(c->device));
-    const int64_t nblk64 = (m.n_cells + ENERGY_BS - 1) / ENERGY_BS;
-    if (nblk64 > (1 << 30)) return set_err(NCE_ERR_ARG, "cell too large");
-    const int nblk = (int)nblk64;
-    const size_t need = (size_t)R * nblk;
-    hipStream_t s = (hipStream_t)stream;
-    double *partial = nullptr;
-    if (int rc1 = partial_for(c, s, need, &partial)) return rc1;
+    const int64_t nb = (m.n_cells + ENERGY_BS - 1) / ENERGY_BS;
+    if (nb > (1 << 30)) return set_err(NCE_ERR_ARG, "cell too large");
+    *nblk = (int)nb;
+    return NCE_OK;
+}
+
+static int ce_energy_launch(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, double *E, int nblk,
+                            double *partial, hipStream_t s) {
     const size_t lds = c->lds_energy[model];
     if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "forward rows do not fit in LDS");
-    if (int rc2 = allow_lds(ce_energy_kernel, lds)) return rc2;
+    if (int rc = allow_lds(ce_energy_kernel, lds)) return rc;
     hipLaunchKernelGGL(ce_energy_kernel, dim3(nblk, R), dim3(ENERGY_BS), lds, s, c->d_models, model, occ, N, partial);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(ce_energy_finish_kernel, dim3(R), dim3(64), 0, s, c->d_models, model, partial, nblk, E);
     HIP_TRY(hipGetLastError());
     return NCE_OK;
+}
+
+extern "C" int nce_ce_energy(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, double *E,
+                             void *stream) {
+    int nblk = 0;
+    if (int rc = ce_energy_blocks(c, model, occ, R, N, E, &nblk)) return rc;
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t s = (hipStream_t)stream;
+    double *partial = nullptr;
+    if (int rc = partial_for(c, s, (size_t)R * nblk, &partial)) return rc;
+    return ce_energy_launch(c, model, occ, R, N, E, nblk, partial, s);
+}
+
+extern "C" int nce_ce_energy_noalloc(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, double *E,
+                                     void *stream) {
+    int nblk = 0;
+    if (int rc = ce_energy_blocks(c, model, occ, R, N, E, &nblk)) return rc;
+    if ((size_t)R * (size_t)nblk > (size_t)NCE_NOALLOC_PARTIALS)
+        return set_err(NCE_ERR_ARG, "n_replicas * ceil(n_cells / 256) = " + std::to_string((size_t)R * nblk) +
+                                        " partial sums exceed the " + std::to_string((size_t)NCE_NOALLOC_PARTIALS) +
+                                        " of the no-allocation scratch");
+    if (!c->noalloc_partial) return set_err(NCE_ERR_HIP, "no-allocation scratch address unknown");
+    HIP_TRY(hipSetDevice(c->device));
+    return ce_energy_launch(c, model, occ, R, N, E, nblk, c->noalloc_partial, (hipStream_t)stream);
 }
 
 template <typename Real, int W>
