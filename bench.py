@@ -119,16 +119,18 @@ def nce_bench(args, cfg):
 
 def served_bench(args):
     """Served request stream: Poisson arrivals of C_v(T) requests, continuous batching on one GPU, against the plain
-    FP64 one-job-at-a-time form on the same requests."""
+    FP64 one-job-at-a-time form on the same requests.  The cell models of the stream's sizes are built before the
+    clock starts (a server keeps the models of the shapes it serves); both forms use them."""
     from neuralce2_amd.serve import Server, job_stream, run_sequential
-    specs = job_stream(args.jobs, args.rate, seed=args.seed, n_min=4, n_max=args.n_max, t_min=16, t_max=64,
-                       n_steps=args.job_steps, chunk=args.chunk)
-    srv = Server(chunk=args.chunk, precision=args.precision or "mixed")
-    srv.run(specs[:2])                                   # warm-up: code objects, model tables, tile cache
+    specs = job_stream(args.jobs, args.rate, seed=args.seed, n_min=4, n_max=args.n_max, t_min=16,
+                       t_max=args.t_max, n_steps=args.job_steps, chunk=args.chunk)
+    srv = Server(chunk=args.chunk, precision=args.precision or "mixed", verify=False)
+    srv.prepare(specs)
+    srv.run(specs[:2])                                   # warm-up: code objects, tile cache
     res, st = srv.run(specs)
     base = None
     if not args.no_cpu:                                  # the plain form is the comparison; skipped with --no-cpu
-        _, base = run_sequential(specs, precision="fp64", chunk=args.chunk)
+        _, base = run_sequential(specs, precision="fp64", chunk=args.chunk, models_from=srv, verify=False)
     return res, st, base, specs
 
 
@@ -172,7 +174,8 @@ def parse():
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rate", type=float, default=100.0, help="requests per second (Poisson)")
     ap.add_argument("--job-steps", type=int, default=100000)
-    ap.add_argument("--n-max", type=int, default=32)
+    ap.add_argument("--n-max", type=int, default=32, help="largest cell: fcc n^3 cubic cells (136: 10,061,824 sites)")
+    ap.add_argument("--t-max", type=int, default=64, help="most temperatures of a request")
     ap.add_argument("--dump-moments", default=None,
                     help="write the per-temperature <E> and C_v of the timed run (npz) for precision comparisons")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -192,26 +195,48 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_per_launch": n, "launches": 3}), flush=True)
 
 
+# the reference's published icet/mchammer CPU sampler at the benchmark's cell size (a different host):
+# results/bench_stage/mchammer.json, n_atoms 1,048,576 (steps_per_s_manual; one process, 900 K, 64^3 cubic cells)
+PUBLISHED_MCHAMMER = {"value": 24.503007275504032, "unit": "swap steps/s (one chain)", "n_sites": 1048576,
+                      "host": "ccc0472.campuscluster.illinois.edu (the reference's CPU node, not this box)",
+                      "source": "results/bench_stage/mchammer.json"}
+
+
 def cpu_baseline(info, occ0, betas, target_s, seed):
-    """FP64 oracle on this host's cores: bounded sample (all replicas, n steps sized to ~target_s)."""
+    """The FP64 oracle on this host's cores on a bounded sample (all replicas, n steps sized to ~target_s), plus a
+    single-thread mchammer-style chain (partner always of the other species) on one replica of the same cell, and the
+    reference's published mchammer figure for this cell size.  Threads: the cores this process may use, at most 16:
+    the box allots 16 CPU cores to a one-GPU job, whatever the machine has."""
     from oracle.ce_oracle import OracleCE
     o = OracleCE(info)
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    cores = min(len(os.sched_getaffinity(0)), 16)
     R = occ0.shape[0]
     occ = np.ascontiguousarray(occ0.copy())
-    E = np.array([o.energy(occ[0])] * R)
+    E0 = o.energy(occ[0])
+    E = np.array([E0] * R)
     t = time.perf_counter()
     o.run_chunk(occ, E, betas, 4, seed=seed, step0=10**9, n_threads=cores)
-    dt = time.perf_counter() - t
-    per_step = max(dt / 4, 1e-6)
+    per_step = max((time.perf_counter() - t) / 4, 1e-6)
     n = int(max(4, min(100000, target_s / per_step)))
     t = time.perf_counter()
     o.run_chunk(occ, E, betas, n, seed=seed, step0=2 * 10**9, n_threads=cores)
     dt = time.perf_counter() - t
+    one = np.ascontiguousarray(occ0[0].copy())
+    n1 = 20000
+    t1 = time.perf_counter()
+    o.mchammer_swaps(one, E0, float(betas[0]), n1, seed=seed)
+    dt1 = time.perf_counter() - t1
+    n1 = int(max(n1, min(2_000_000, n1 * 0.25 * target_s / max(dt1, 1e-9))))
+    t1 = time.perf_counter()
+    o.mchammer_swaps(one, E0, float(betas[0]), n1, seed=seed + 1)
+    dt1 = time.perf_counter() - t1
     return dict(value=R * n / dt, unit="MC swap attempts/s", cores=cores, kind="port",
-                sample=f"{R} replicas x {n} steps of the same cell and ladder, FP64 oracle, OpenMP over replicas "
-                       f"({dt:.1f} s)")
+                sample=f"{R} replicas x {n} steps of the same cell and ladder, FP64 oracle, OpenMP over replicas on "
+                       f"{cores} threads ({dt:.1f} s)",
+                single_thread_mchammer_style={"value": n1 / dt1, "unit": "swap steps/s (one chain, one thread)",
+                                              "sample": f"{n1} swaps of one replica, partner of the other species, "
+                                                        f"FP64 local energy change ({dt1:.1f} s)"},
+                published_mchammer=PUBLISHED_MCHAMMER)
 
 
 def main():
@@ -251,7 +276,7 @@ def main():
                 "scaling": "weak", "vs_baseline": None, "dtype": "f32 dE / f64 totals",
                 "data": "synthetic request stream (seeded)",
                 "config": {"workload": f"{args.jobs} C_v(T) requests, Poisson {args.rate}/s, fcc n^3 cubic cells "
-                                       f"log-uniform 256-{4 * args.n_max ** 3} sites, x in 0.1..0.9, 16-64 "
+                                       f"log-uniform 256-{4 * args.n_max ** 3} sites, x in 0.1..0.9, 16-{args.t_max} "
                                        f"temperatures, {args.job_steps} steps each, chunk {args.chunk}"},
                 "latency_p50_s": st["latency_p50_s"], "latency_p99_s": st["latency_p99_s"],
                 "wall_s": st["wall_s"], "attempts": st["attempts"],

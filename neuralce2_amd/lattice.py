@@ -166,17 +166,19 @@ def lattice_coordinates(prim, atoms, tol=5e-3):
     The basis is the one whose nearest lattice point is closest in Cartesian distance; a site farther than `tol`
     Angstrom from every lattice point raises OffLatticeError.  n is not reduced into the supercell."""
     P = prim.cell
-    frac = np.linalg.solve(P.T, atoms.positions.T).T
+    frac = atoms.positions @ np.linalg.inv(P)
     fb = prim.get_scaled_positions()
-    best = np.full(len(atoms), np.inf)
-    basis = np.zeros(len(atoms), np.int64)
-    n = np.zeros((len(atoms), 3), np.int64)
+    best = basis = n = None
     for b in range(len(prim)):
         r = frac - fb[b]
         nb = np.rint(r)
-        miss = np.linalg.norm((r - nb) @ P, axis=1)
+        r -= nb
+        miss = np.sqrt(np.einsum("ij,ij->i", r @ P, r @ P))
+        if b == 0:
+            best, basis, n = miss, np.zeros(len(atoms), np.int64), nb.astype(np.int64)
+            continue
         take = miss < best
-        best[take] = miss[take]
+        best = np.where(take, miss, best)
         basis[take] = b
         n[take] = nb[take].astype(np.int64)
     if len(atoms) and best.max() > tol:

@@ -315,3 +315,38 @@ int oracle_enumerate_energies(int32_t n, int32_t k, int64_t n_terms, int32_t mo,
     }
     return 0;
 }
+
+/* ── single-thread restatement of mchammer's canonical swap step (the reference's CPU sampler benchmark,
+ * scripts/bench/bench_stage_mchammer.py:97-120: CanonicalEnsemble.get_swapped_state / _get_property_change /
+ * _acceptance_condition / update_occupations) on a binary one-sublattice cell: site a uniform, site b uniform among
+ * the sites of the other species (a swap always changes the configuration), the local energy change of the two
+ * sites, Metropolis at beta, occupations and per-species site lists updated on acceptance.  Uniforms from Philox
+ * counter (t, 0, 0x4D43, 0) under the key (seed).  Returns accepted swaps; E is advanced. */
+int64_t oracle_mchammer_swaps(const oracle_ce *m, int8_t *occ, int64_t N, int64_t n_steps, double beta,
+                              uint64_t seed, double *E) {
+    int64_t *list[2], cnt[2] = {0, 0};
+    int64_t *where = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+    list[0] = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+    list[1] = (int64_t *)malloc(sizeof(int64_t) * (size_t)N);
+    if (!where || !list[0] || !list[1]) { free(where); free(list[0]); free(list[1]); return -1; }
+    for (int64_t i = 0; i < N; ++i) { const int s = occ[i] & 1; where[i] = cnt[s]; list[s][cnt[s]++] = i; }
+    int64_t acc = 0;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (int64_t t = 0; t < n_steps && cnt[0] > 0 && cnt[1] > 0; ++t) {
+        uint32_t ctr[4] = {(uint32_t)t, (uint32_t)((uint64_t)t >> 32), 0x4D43u, 0u}, w[4];
+        oracle_philox4x32_10(ctr, key, w);
+        const int64_t a = (int64_t)(((uint64_t)w[0] * (uint64_t)N) >> 32);
+        const int sa = occ[a] & 1, sb = 1 - sa;
+        const int64_t b = list[sb][(int64_t)(((uint64_t)w[1] * (uint64_t)cnt[sb]) >> 32)];
+        const double u = ((double)((((uint64_t)w[3] << 32) | w[2]) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double dE = oracle_ce_delta(m, occ, a, b, sb, sa);
+        if (log(u) < -beta * dE) {
+            occ[a] = (int8_t)sb; occ[b] = (int8_t)sa; *E += dE; ++acc;
+            const int64_t wa = where[a], wb = where[b];       /* a moves to list sb at b's slot, b to list sa */
+            list[sb][wb] = a; where[a] = wb;
+            list[sa][wa] = b; where[b] = wa;
+        }
+    }
+    free(where); free(list[0]); free(list[1]);
+    return acc;
+}

@@ -57,6 +57,8 @@ def _lib():
     lib.oracle_draw.restype = None
     lib.oracle_enumerate_energies.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int32, _p, _p, C.c_double, _p]
     lib.oracle_enumerate_energies.restype = C.c_int
+    lib.oracle_mchammer_swaps.argtypes = [_p, _p, C.c_int64, C.c_int64, C.c_double, C.c_uint64, _p]
+    lib.oracle_mchammer_swaps.restype = C.c_int64
     return lib
 
 
@@ -149,6 +151,16 @@ class OracleCE:
             c.inj_ia = P(ia); c.inj_ib = P(ib); c.inj_u = P(u)
         lib().oracle_pt_chunk(C.addressof(self.s), C.addressof(c), int(n_steps), int(n_threads))
         return n_acc, e_min
+
+    def mchammer_swaps(self, occ, E, beta, n_steps, seed=0):
+        """single-thread canonical swaps of one configuration in the manner of mchammer's CanonicalEnsemble (the
+        partner always of the other species); occ (N,) int8 updated in place; returns (accepted, new E)."""
+        assert occ.dtype == np.int8 and occ.flags.c_contiguous
+        e = np.array([float(E)])
+        acc = lib().oracle_mchammer_swaps(C.addressof(self.s), occ.ctypes.data, len(occ), int(n_steps), float(beta),
+                                          int(seed), e.ctypes.data)
+        assert acc >= 0
+        return int(acc), float(e[0])
 
 
 def exchange(E, rep_of_slot, beta_slot, beta_rep, seed=0, ex_index=0, ladder_id=None, inj_parity=-1, inj_u=None):

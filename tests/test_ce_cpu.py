@@ -304,3 +304,15 @@ def test_exact_enumeration_energies_match_brute_force():
         assert abs(E[r] - brute_energy(cs, params, a, float(N))) < 1e-12, r
     # the energies take few distinct values per symmetry class: the minimum is reached at least N / |orbit| times
     assert np.sum(np.isclose(E, E.min(), rtol=0, atol=1e-12)) >= 2
+
+
+def test_mchammer_style_swaps_keep_the_energy_and_composition(ptni_ce, ptni_template, ptni_atoms_256):
+    """the single-thread CPU sampler (timed beside the GPU as the mchammer-style baseline): every proposal changes
+    the configuration, the composition is kept, and the carried energy equals a full recompute."""
+    o, sig, _ = _oracle(ptni_ce, ptni_template, ptni_atoms_256)
+    occ = sig.copy()
+    E0 = o.energy(occ)
+    acc, E = o.mchammer_swaps(occ, E0, 1.0 / (8.617333e-5 * 900.0), 5000, seed=3)
+    assert 0 < acc < 5000
+    assert np.bincount(occ, minlength=2).tolist() == np.bincount(sig, minlength=2).tolist()
+    assert abs(E - o.energy(occ)) < 1e-9
