@@ -101,6 +101,8 @@ struct ModelDev {
     // speculative kernel, 64 entries per iteration in three classes (FsLo / FsFull / FsTail), per precision
     int32_t fs_ok, fs_nlo, fs_nfull, fs_ntail;
     const unsigned char *fs_tab32, *fs_tab64;
+    // the same FP64 layout with every |coefficient| divided by its cluster's order (full energies)
+    const unsigned char *fs_tab_e;
 };
 
 // NeuralCE model on the device (kernels in nn_update.hip)
@@ -1253,6 +1255,90 @@ __global__ __launch_bounds__(ENERGY_BS) void ce_energy_kernel(const ModelDev *__
     }
 }
 
+// Full energies of the binary fast path, one wave per site at a time.  A cluster is an entry of each of its
+// members, so E = zerolet + factor * sum_x (-1)^[s_x = 0] * sum_e (|c_e| / order_e) (-1)^parity(bits_x & m_e): the
+// window-parity sum of the chunk kernel with each coefficient divided by its cluster's order (window bits
+// [species == 0], the centre bit set as the carrier of each coefficient's sign).  Each block takes EF_SITES
+// consecutive sites, 64 per wave: consecutive windows overlap in the caches, and the cell coordinate is stepped
+// rather than divided.  Same partial layout as ce_energy_kernel (blocks of 256 cells, nB = 1).
+constexpr int EF_BS = 256;
+constexpr int EF_SITES = 256;
+static_assert(EF_SITES == ENERGY_BS, "partial sums are shared with ce_energy_finish_kernel");
+
+__global__ __launch_bounds__(EF_BS) void ce_energy_fast_kernel(const ModelDev *__restrict__ models, int model,
+                                                               const int8_t *__restrict__ occ_all, int64_t N,
+                                                               double *__restrict__ partial) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ModelDev &mg = models[model];
+    const int nlo = mg.fs_nlo, nfull = mg.fs_nfull, ntail = mg.fs_ntail, Wn = mg.W_b[0];
+    const int d0 = mg.d0, d1 = mg.d1, d2 = mg.d2;
+    const size_t fsb = fs_bytes<double>(nlo, nfull, ntail);
+    {
+        const uint4 *src = (const uint4 *)mg.fs_tab_e;
+        uint4 *dst = (uint4 *)smem;
+        for (int i = threadIdx.x; i < (int)(fsb / 16); i += EF_BS) dst[i] = src[i];
+    }
+    double *s_red = (double *)(smem + align16(fsb));
+    const FsLo<double> *s_lo = (const FsLo<double> *)smem;
+    const FsFull *s_full = (const FsFull *)(s_lo + 64 * nlo);
+    const FsTail *s_tail = (const FsTail *)(s_full + 64 * nfull);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int4 wofs[3];
+    bool wok[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int slot = lane + 64 * j;
+        wok[j] = slot < Wn;
+        wofs[j] = wok[j] ? mg.win[slot] : make_int4(0, 0, 0, 0);
+    }
+    __syncthreads();
+    const NCE_GLOBAL int8_t *occ = gptr(occ_all) + (int64_t)blockIdx.y * N;
+    const int64_t x0 = (int64_t)blockIdx.x * EF_SITES + 64 * w;
+    Coord c = site_coord(mg, x0 < N ? x0 : 0);
+    double acc = 0.0;
+    for (int i = 0; i < 64 && x0 + i < N; ++i) {
+        unsigned long long Wd[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            int sp = 1;
+            if (wok[j]) sp = occ[window_site(mg, c, wofs[j])];
+            Wd[j] = __ballot(wok[j] && sp == 0);
+        }
+        Wd[0] |= 1ull;
+        double a = 0.0;
+        const uint32_t w32 = (uint32_t)Wd[0], whi = (uint32_t)(Wd[0] >> 32);
+        for (int it = 0; it < nlo; ++it) {
+            const FsLo<double> e = s_lo[64 * it + lane];
+            a += fs_term((uint64_t)e.c, (uint32_t)__builtin_popcount(w32 & e.m));
+        }
+        for (int it = 0; it < nfull; ++it) {
+            const FsFull e = s_full[64 * it + lane];
+            a += fs_term((uint64_t)e.c,
+                         (uint32_t)__builtin_popcount(and_xor(w32, (uint32_t)e.m, whi & (uint32_t)(e.m >> 32))));
+        }
+        for (int it = 0; it < ntail; ++it) {
+            const FsTail e = s_tail[64 * it + lane];
+            const unsigned long long w1 = e.word == 1 ? Wd[1] : Wd[2];
+            const uint32_t f = and_xor(w32, (uint32_t)e.m0, and_xor(whi, (uint32_t)(e.m0 >> 32),
+                                       and_xor((uint32_t)w1, (uint32_t)e.mk, (uint32_t)(w1 >> 32) & (uint32_t)(e.mk >> 32))));
+            a += fs_term((uint64_t)e.c, (uint32_t)__builtin_popcount(f));
+        }
+        const double S = wave_sum(a);
+        acc += occ[x0 + i] == 0 ? -S : S;
+        if (++c.k2 == d2) {                            // next site (one basis): step the cell coordinate
+            c.k2 = 0;
+            if (++c.k1 == d1) {
+                c.k1 = 0;
+                if (++c.k0 == d0) c.k0 = 0;
+            }
+        }
+    }
+    if (lane == 0) s_red[w] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+}
+
 __global__ void ce_energy_finish_kernel(const ModelDev *__restrict__ models, int model,
                                         const double *__restrict__ partial, int n_part, double *E) {
     const ModelDev m = models[model];
@@ -1436,7 +1522,7 @@ static void magic_u32(uint32_t d, uint32_t *m, uint32_t *sh) {
 static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m) {
     m.fs_ok = 0;
     m.fs_nlo = m.fs_nfull = m.fs_ntail = 0;
-    m.fs_tab32 = m.fs_tab64 = nullptr;
+    m.fs_tab32 = m.fs_tab64 = m.fs_tab_e = nullptr;
     if (d->n_basis != 1 || d->n_active != 1 || d->active_nbasis[0] != 1 || d->max_window > 192) return NCE_OK;
     for (int sl = 0; sl < d->n_sublattices; ++sl) {
         const double *pf = d->pf + sl * 4;
@@ -1446,16 +1532,17 @@ static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m
     struct Ent {
         unsigned long long m0, mk;
         double c;
-        int word;
+        int word, order;
     };
     std::vector<Ent> lo, full, tail;
     const int Jb = d->entries_per_basis[0];
     for (int j = 0; j < Jb; ++j) {
-        Ent e{0ull, 0ull, d->entry_coef[j], 0};
+        Ent e{0ull, 0ull, d->entry_coef[j], 0, 0};
         int centre = 0;
         for (int q = 0; q < 4; ++q) {
             const uint32_t sl = d->entry_slot[(size_t)j * 4 + q];
             if (sl == 0xFFFFu) continue;
+            ++e.order;
             if (sl == 0) { ++centre; continue; }
             const int word = (int)(sl >> 6);
             const unsigned long long bit = 1ull << (sl & 63);
@@ -1475,7 +1562,8 @@ static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m
     }
     const int nlo = (int)((lo.size() + 63) / 64), nfull = (int)((full.size() + 63) / 64),
               ntail = (int)((tail.size() + 63) / 64);
-    std::vector<unsigned char> t32(fs_bytes<float>(nlo, nfull, ntail), 0), t64(fs_bytes<double>(nlo, nfull, ntail), 0);
+    std::vector<unsigned char> t32(fs_bytes<float>(nlo, nfull, ntail), 0), t64(fs_bytes<double>(nlo, nfull, ntail), 0),
+        te(fs_bytes<double>(nlo, nfull, ntail), 0);
     auto bits32 = [](double v) { float f = (float)v; uint32_t u; std::memcpy(&u, &f, 4); return u; };
     auto bits64 = [](double v) { uint64_t u; std::memcpy(&u, &v, 8); return u; };
     {
@@ -1501,10 +1589,19 @@ static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m
             u64[i].c = bits64(tail[i].c);
         }
         for (size_t i = tail.size(); i < (size_t)64 * ntail; ++i) u32[i].word = u64[i].word = 1;
+        // full-energy table: every cluster is an entry of each of its members, so |c| / order counts it once
+        std::memcpy(te.data(), t64.data(), te.size());
+        FsLo<double> *le = (FsLo<double> *)te.data();
+        for (size_t i = 0; i < lo.size(); ++i) le[i].c = bits64(lo[i].c / lo[i].order);
+        FsFull *fe = (FsFull *)(le + 64 * nlo);
+        for (size_t i = 0; i < full.size(); ++i) fe[i].c = bits64(full[i].c / full[i].order);
+        FsTail *ue = (FsTail *)(fe + 64 * nfull);
+        for (size_t i = 0; i < tail.size(); ++i) ue[i].c = bits64(tail[i].c / tail[i].order);
     }
     if (!t32.empty()) {
         UP(t32.data(), t32.size(), &m.fs_tab32);
         UP(t64.data(), t64.size(), &m.fs_tab64);
+        UP(te.data(), te.size(), &m.fs_tab_e);
     }
     m.fs_nlo = nlo; m.fs_nfull = nfull; m.fs_ntail = ntail;
     m.fs_ok = 1;
@@ -1653,6 +1750,19 @@ static int ce_energy_blocks(nce_ctx *c, int32_t model, const int8_t *occ, int32_
 
 static int ce_energy_launch(nce_ctx *c, int32_t model, const int8_t *occ, int32_t R, int64_t N, double *E, int nblk,
                             double *partial, hipStream_t s) {
+    const ModelDev &m = c->models[model];
+    const char *fe = std::getenv("NCE_CE_ENERGY_ROWS");              // tests: the forward-row kernel on any model
+    if (m.fs_ok && m.nB == 1 && m.fs_tab_e && !(fe && fe[0] == '1')) {
+        const size_t lds = align16(fs_bytes<double>(m.fs_nlo, m.fs_nfull, m.fs_ntail)) + 4 * sizeof(double);
+        if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "entry tables do not fit in LDS");
+        if (int rc = allow_lds(ce_energy_fast_kernel, lds)) return rc;
+        hipLaunchKernelGGL(ce_energy_fast_kernel, dim3(nblk, R), dim3(EF_BS), lds, s, c->d_models, model, occ, N,
+                           partial);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(ce_energy_finish_kernel, dim3(R), dim3(64), 0, s, c->d_models, model, partial, nblk, E);
+        HIP_TRY(hipGetLastError());
+        return NCE_OK;
+    }
     const size_t lds = c->lds_energy[model];
     if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "forward rows do not fit in LDS");
     if (int rc = allow_lds(ce_energy_kernel, lds)) return rc;

@@ -354,3 +354,32 @@ def test_pt_statistics_match_exact_enumeration(torch_dev, precision):
     print(f"precision {precision}: max |<E> - exact| {dE.max():.2e} eV/site, C_v ratio "
           f"{np.round(C_mc / ex_C, 3).tolist()}")
     assert dE.max() < 3e-4 and dC.max() < 0.06, (dE, C_mc / ex_C)
+
+
+@pytest.mark.parametrize("n,x", [(16, 0.3), (64, 0.25), (3, 0.5)])
+def test_parity_energy_kernel_equals_forward_rows(torch_dev, n, x, monkeypatch):
+    """full energies of the binary fast path (window-parity sums with coefficients / order) against the
+    forward-row kernel (every cluster of every cell), on cells of 108 to 1,048,576 sites."""
+    import time
+    import torch
+    from neuralce2_amd.synthetic import synthetic_ce, fcc_cell
+    from neuralce2_amd.ce.classical_ce_gpu import GPUClusterExpansion
+    at = fcc_cell(n, x, 2)
+    g = GPUClusterExpansion(synthetic_ce(0), at, energy_scale=float(len(at)))
+    sig = g.encode(at).astype(np.int8)
+    rng = np.random.default_rng(n)
+    R = 8
+    occ = torch.tensor(np.stack([rng.permutation(sig) for _ in range(R)]), device=torch_dev)
+    E_fast = torch.empty(R, dtype=torch.float64, device=torch_dev)
+    E_rows = torch.empty_like(E_fast)
+    for _ in range(2):                                   # second round timed (code objects loaded)
+        torch.cuda.synchronize(); t0 = time.perf_counter()
+        g.ctx.ce_energy(g.model, occ, E_fast)
+        torch.cuda.synchronize(); t_fast = time.perf_counter() - t0
+        monkeypatch.setenv("NCE_CE_ENERGY_ROWS", "1")
+        torch.cuda.synchronize(); t0 = time.perf_counter()
+        g.ctx.ce_energy(g.model, occ, E_rows)
+        torch.cuda.synchronize(); t_rows = time.perf_counter() - t0
+        monkeypatch.delenv("NCE_CE_ENERGY_ROWS")
+    print(f"{len(at)} sites x {R}: parity kernel {t_fast * 1e3:.2f} ms, forward rows {t_rows * 1e3:.2f} ms")
+    assert torch.max(torch.abs(E_fast - E_rows)).item() / len(at) < 1e-12
