@@ -101,7 +101,8 @@ struct ModelDev {
     // speculative kernel, 64 entries per iteration in three classes (FsLo / FsFull / FsTail), per precision
     int32_t fs_ok, fs_nlo, fs_nfull, fs_ntail;
     const unsigned char *fs_tab32, *fs_tab64;
-    // the same FP64 layout with every |coefficient| divided by its cluster's order (full energies)
+    // full energies: the FP64 layout over one entry per cluster (the entry centred on the cluster's first member)
+    int32_t fse_nlo, fse_nfull, fse_ntail;
     const unsigned char *fs_tab_e;
 };
 
@@ -1255,12 +1256,14 @@ __global__ __launch_bounds__(ENERGY_BS) void ce_energy_kernel(const ModelDev *__
     }
 }
 
-// Full energies of the binary fast path, one wave per site at a time.  A cluster is an entry of each of its
-// members, so E = zerolet + factor * sum_x (-1)^[s_x = 0] * sum_e (|c_e| / order_e) (-1)^parity(bits_x & m_e): the
-// window-parity sum of the chunk kernel with each coefficient divided by its cluster's order (window bits
-// [species == 0], the centre bit set as the carrier of each coefficient's sign).  Each block takes EF_SITES
-// consecutive sites, 64 per wave: consecutive windows overlap in the caches, and the cell coordinate is stepped
-// rather than divided.  Same partial layout as ce_energy_kernel (blocks of 256 cells, nB = 1).
+// Full energies of the binary fast path, one wave per site at a time.  Every cluster is counted once, by the entry
+// centred on its first member: E = zerolet + factor * sum_x (-1)^[s_x = 0] * sum_e |c_e| (-1)^parity(bits_x & m_e)
+// over those entries, the window-parity sum of the chunk kernel (window bits [species == 0], the centre bit set as
+// the carrier of each coefficient's sign).  Each block takes EF_SITES consecutive sites, 64 per wave: consecutive
+// windows overlap in the caches, and the cell coordinate is stepped rather than divided.  The next site's window
+// is loaded while the current one is summed; the centre's species is bit 0 of its window (slot 0 is the site), and
+// each lane keeps its own signed sum over the wave's sites, reduced once at the end.  Same partial layout as
+// ce_energy_kernel (blocks of 256 cells, nB = 1).
 constexpr int EF_BS = 256;
 constexpr int EF_SITES = 256;
 static_assert(EF_SITES == ENERGY_BS, "partial sums are shared with ce_energy_finish_kernel");
@@ -1270,7 +1273,7 @@ __global__ __launch_bounds__(EF_BS) void ce_energy_fast_kernel(const ModelDev *_
                                                                double *__restrict__ partial) {
     extern __shared__ __align__(16) unsigned char smem[];
     const ModelDev &mg = models[model];
-    const int nlo = mg.fs_nlo, nfull = mg.fs_nfull, ntail = mg.fs_ntail, Wn = mg.W_b[0];
+    const int nlo = mg.fse_nlo, nfull = mg.fse_nfull, ntail = mg.fse_ntail, Wn = mg.W_b[0];
     const int d0 = mg.d0, d1 = mg.d1, d2 = mg.d2;
     const size_t fsb = fs_bytes<double>(nlo, nfull, ntail);
     {
@@ -1294,16 +1297,29 @@ __global__ __launch_bounds__(EF_BS) void ce_energy_fast_kernel(const ModelDev *_
     __syncthreads();
     const NCE_GLOBAL int8_t *occ = gptr(occ_all) + (int64_t)blockIdx.y * N;
     const int64_t x0 = (int64_t)blockIdx.x * EF_SITES + 64 * w;
+    const int n_here = x0 < N ? (int)(N - x0 < 64 ? N - x0 : 64) : 0;
     Coord c = site_coord(mg, x0 < N ? x0 : 0);
+    int sp_next[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sp_next[j] = (wok[j] && n_here > 0) ? occ[window_site(mg, c, wofs[j])] : 1;
     double acc = 0.0;
-    for (int i = 0; i < 64 && x0 + i < N; ++i) {
+    for (int i = 0; i < n_here; ++i) {
         unsigned long long Wd[3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            int sp = 1;
-            if (wok[j]) sp = occ[window_site(mg, c, wofs[j])];
-            Wd[j] = __ballot(wok[j] && sp == 0);
+        for (int j = 0; j < 3; ++j) Wd[j] = __ballot(wok[j] && sp_next[j] == 0);
+        if (++c.k2 == d2) {                            // next site (one basis): step the cell coordinate
+            c.k2 = 0;
+            if (++c.k1 == d1) {
+                c.k1 = 0;
+                if (++c.k0 == d0) c.k0 = 0;
+            }
         }
+        if (i + 1 < n_here) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (wok[j]) sp_next[j] = occ[window_site(mg, c, wofs[j])];
+        }
+        const bool centre0 = (Wd[0] & 1ull) != 0;      // slot 0 is the site itself
         Wd[0] |= 1ull;
         double a = 0.0;
         const uint32_t w32 = (uint32_t)Wd[0], whi = (uint32_t)(Wd[0] >> 32);
@@ -1323,16 +1339,9 @@ __global__ __launch_bounds__(EF_BS) void ce_energy_fast_kernel(const ModelDev *_
                                        and_xor((uint32_t)w1, (uint32_t)e.mk, (uint32_t)(w1 >> 32) & (uint32_t)(e.mk >> 32))));
             a += fs_term((uint64_t)e.c, (uint32_t)__builtin_popcount(f));
         }
-        const double S = wave_sum(a);
-        acc += occ[x0 + i] == 0 ? -S : S;
-        if (++c.k2 == d2) {                            // next site (one basis): step the cell coordinate
-            c.k2 = 0;
-            if (++c.k1 == d1) {
-                c.k1 = 0;
-                if (++c.k0 == d0) c.k0 = 0;
-            }
-        }
+        acc += centre0 ? -a : a;
     }
+    acc = wave_sum(acc);
     if (lane == 0) s_red[w] = acc;
     __syncthreads();
     if (threadIdx.x == 0)
@@ -1522,6 +1531,7 @@ static void magic_u32(uint32_t d, uint32_t *m, uint32_t *sh) {
 static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m) {
     m.fs_ok = 0;
     m.fs_nlo = m.fs_nfull = m.fs_ntail = 0;
+    m.fse_nlo = m.fse_nfull = m.fse_ntail = 0;
     m.fs_tab32 = m.fs_tab64 = m.fs_tab_e = nullptr;
     if (d->n_basis != 1 || d->n_active != 1 || d->active_nbasis[0] != 1 || d->max_window > 192) return NCE_OK;
     for (int sl = 0; sl < d->n_sublattices; ++sl) {
@@ -1532,17 +1542,19 @@ static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m
     struct Ent {
         unsigned long long m0, mk;
         double c;
-        int word, order;
+        int word;
     };
-    std::vector<Ent> lo, full, tail;
+    // lo / full / tail: every entry (swap energy changes); the *_e lists: the entries whose centre is member 0 of
+    // their cluster (members are listed in the cluster's own order, so that member sits in window slot 0), one
+    // per cluster and translation, which is what a full energy sums
+    std::vector<Ent> lo, full, tail, lo_e, full_e, tail_e;
     const int Jb = d->entries_per_basis[0];
     for (int j = 0; j < Jb; ++j) {
-        Ent e{0ull, 0ull, d->entry_coef[j], 0, 0};
+        Ent e{0ull, 0ull, d->entry_coef[j], 0};
         int centre = 0;
         for (int q = 0; q < 4; ++q) {
             const uint32_t sl = d->entry_slot[(size_t)j * 4 + q];
             if (sl == 0xFFFFu) continue;
-            ++e.order;
             if (sl == 0) { ++centre; continue; }
             const int word = (int)(sl >> 6);
             const unsigned long long bit = 1ull << (sl & 63);
@@ -1556,14 +1568,24 @@ static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m
         if (centre != 1) return NCE_OK;
         if (e.c == 0.0) continue;
         if (e.c < 0.0) { e.m0 |= 1ull; e.c = -e.c; }
-        if (e.word) tail.push_back(e);
-        else if ((e.m0 >> 32) == 0) lo.push_back(e);
-        else full.push_back(e);
+        const bool first = d->entry_slot[(size_t)j * 4] == 0;
+        if (e.word) {
+            tail.push_back(e);
+            if (first) tail_e.push_back(e);
+        } else if ((e.m0 >> 32) == 0) {
+            lo.push_back(e);
+            if (first) lo_e.push_back(e);
+        } else {
+            full.push_back(e);
+            if (first) full_e.push_back(e);
+        }
     }
     const int nlo = (int)((lo.size() + 63) / 64), nfull = (int)((full.size() + 63) / 64),
               ntail = (int)((tail.size() + 63) / 64);
+    const int nlo_e = (int)((lo_e.size() + 63) / 64), nfull_e = (int)((full_e.size() + 63) / 64),
+              ntail_e = (int)((tail_e.size() + 63) / 64);
     std::vector<unsigned char> t32(fs_bytes<float>(nlo, nfull, ntail), 0), t64(fs_bytes<double>(nlo, nfull, ntail), 0),
-        te(fs_bytes<double>(nlo, nfull, ntail), 0);
+        te(fs_bytes<double>(nlo_e, nfull_e, ntail_e), 0);
     auto bits32 = [](double v) { float f = (float)v; uint32_t u; std::memcpy(&u, &f, 4); return u; };
     auto bits64 = [](double v) { uint64_t u; std::memcpy(&u, &v, 8); return u; };
     {
@@ -1589,21 +1611,33 @@ static int build_fast_tables(nce_ctx *c, const nce_ce_model_desc *d, ModelDev &m
             u64[i].c = bits64(tail[i].c);
         }
         for (size_t i = tail.size(); i < (size_t)64 * ntail; ++i) u32[i].word = u64[i].word = 1;
-        // full-energy table: every cluster is an entry of each of its members, so |c| / order counts it once
-        std::memcpy(te.data(), t64.data(), te.size());
+        // full-energy table, FP64 layout over the first-member entries
         FsLo<double> *le = (FsLo<double> *)te.data();
-        for (size_t i = 0; i < lo.size(); ++i) le[i].c = bits64(lo[i].c / lo[i].order);
-        FsFull *fe = (FsFull *)(le + 64 * nlo);
-        for (size_t i = 0; i < full.size(); ++i) fe[i].c = bits64(full[i].c / full[i].order);
-        FsTail *ue = (FsTail *)(fe + 64 * nfull);
-        for (size_t i = 0; i < tail.size(); ++i) ue[i].c = bits64(tail[i].c / tail[i].order);
+        for (size_t i = 0; i < lo_e.size(); ++i) {
+            le[i].m = (uint32_t)lo_e[i].m0;
+            le[i].c = bits64(lo_e[i].c);
+        }
+        FsFull *fe = (FsFull *)(le + 64 * nlo_e);
+        for (size_t i = 0; i < full_e.size(); ++i) {
+            fe[i].m = full_e[i].m0;
+            fe[i].c = bits64(full_e[i].c);
+        }
+        FsTail *ue = (FsTail *)(fe + 64 * nfull_e);
+        for (size_t i = 0; i < tail_e.size(); ++i) {
+            ue[i].m0 = tail_e[i].m0;
+            ue[i].mk = tail_e[i].mk;
+            ue[i].word = (uint32_t)tail_e[i].word;
+            ue[i].c = bits64(tail_e[i].c);
+        }
+        for (size_t i = tail_e.size(); i < (size_t)64 * ntail_e; ++i) ue[i].word = 1;
     }
     if (!t32.empty()) {
         UP(t32.data(), t32.size(), &m.fs_tab32);
         UP(t64.data(), t64.size(), &m.fs_tab64);
-        UP(te.data(), te.size(), &m.fs_tab_e);
     }
+    if (!te.empty()) UP(te.data(), te.size(), &m.fs_tab_e);
     m.fs_nlo = nlo; m.fs_nfull = nfull; m.fs_ntail = ntail;
+    m.fse_nlo = nlo_e; m.fse_nfull = nfull_e; m.fse_ntail = ntail_e;
     m.fs_ok = 1;
     return NCE_OK;
 }
@@ -1753,7 +1787,7 @@ static int ce_energy_launch(nce_ctx *c, int32_t model, const int8_t *occ, int32_
     const ModelDev &m = c->models[model];
     const char *fe = std::getenv("NCE_CE_ENERGY_ROWS");              // tests: the forward-row kernel on any model
     if (m.fs_ok && m.nB == 1 && m.fs_tab_e && !(fe && fe[0] == '1')) {
-        const size_t lds = align16(fs_bytes<double>(m.fs_nlo, m.fs_nfull, m.fs_ntail)) + 4 * sizeof(double);
+        const size_t lds = align16(fs_bytes<double>(m.fse_nlo, m.fse_nfull, m.fse_ntail)) + 4 * sizeof(double);
         if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "entry tables do not fit in LDS");
         if (int rc = allow_lds(ce_energy_fast_kernel, lds)) return rc;
         hipLaunchKernelGGL(ce_energy_fast_kernel, dim3(nblk, R), dim3(EF_BS), lds, s, c->d_models, model, occ, N,
@@ -1912,10 +1946,11 @@ static int plan_chunk(nce_ctx *c, const nce_pt_job *jobs, int32_t n_jobs, int64_
     const int per_cu = (int)((items.size() + c->n_cu - 1) / c->n_cu);
     // general kernel: waves cooperating on one step of a replica; speculative kernel: steps evaluated in parallel
     int W = tile == 0 ? (per_cu >= 4 ? 1 : per_cu >= 2 ? 2 : 4) : (tile > 8 ? 8 : tile);
-    // default speculative tile: fewer waves when several replicas share a CU, and when the windows of a group's
-    // swaps would overlap often (expected pairs W(W-1)/2 * 4 * window / N of the smallest cell above 2: each
-    // overlap with an accepted earlier swap costs a re-evaluation)
-    int tile_spec = (tile == 4 || tile == 8 || tile == 16) ? tile : (per_cu >= 4 ? 4 : per_cu >= 2 ? 8 : 16);
+    // default speculative tile: 8 waves when several replicas share a CU (16 at one), fewer when the windows of a
+    // group's swaps would overlap often (expected pairs W(W-1)/2 * 4 * window / N of the smallest cell above 2: each
+    // overlap with an accepted earlier swap costs a re-evaluation).  Measured on cells of 2,048 to 1,048,576 sites
+    // at 1 to 12 replicas per CU, 8 waves beat 4 wherever the overlap bound allows 8.
+    int tile_spec = (tile == 4 || tile == 8 || tile == 16) ? tile : (per_cu >= 2 ? 8 : 16);
     if (tile == 0) {
         double dens = 0.0;                             // largest window / sites over the jobs
         for (int j = 0; j < n_jobs; ++j) {

@@ -44,7 +44,9 @@ def test_results_same_alone_and_in_the_mix(torch_dev):
              JobSpec(2, n=3, x=0.75, n_temps=6, n_steps=4000, seed=13),
              JobSpec(3, n=8, x=0.4, n_temps=16, n_steps=2000, seed=14),
              JobSpec(4, n=3, x=0.5, n_temps=4, n_steps=2000, seed=15, kind="nce"),
-             JobSpec(5, n=5, x=0.3, n_temps=10, n_steps=1000, seed=16)]
+             JobSpec(5, n=5, x=0.3, n_temps=10, n_steps=1000, seed=16),
+             JobSpec(6, n=14, x=0.5, n_temps=6, n_steps=2000, seed=17, arrival=0.0),
+             JobSpec(7, n=24, x=0.3, n_temps=4, n_steps=3000, seed=18)]
     srv = Server(chunk=1000, keep_occ=True, nce_model=_ptni_factory())
     mix, stats = srv.run(specs)
     assert stats["n_jobs"] == len(specs) and stats["attempts"] == sum(s.n_steps * s.n_temps for s in specs)
