@@ -133,15 +133,21 @@ class PTJobState:
         self.R = R = len(betas_slot)
         self.N = gce.n_atoms
         sig = np.asarray(sigma0)
-        if sig.ndim == 1:
-            sig = np.broadcast_to(sig, (R, self.N))
-        if sig.shape != (R, self.N):
-            raise ValueError(f"initial sigma shape {sig.shape} != ({R}, {self.N})")
+        if sig.shape != (R, self.N) and sig.shape != (self.N,):
+            raise ValueError(f"initial sigma shape {sig.shape} != ({R}, {self.N}) or ({self.N},)")
         if sig.min() < 0 or sig.max() > 1:
             raise ValueError("species index out of range for a binary cluster expansion")
-        self.occ = torch.tensor(np.ascontiguousarray(sig, np.int8), device=dev)
         self.energy = torch.empty(R, dtype=torch.float64, device=dev)
-        gce.ctx.ce_energy(gce.model, self.occ, self.energy)
+        if sig.ndim == 1:
+            # every replica starts from the same configuration: one row uploaded and replicated on the device, its
+            # FP64 energy computed once
+            row = torch.tensor(np.ascontiguousarray(sig, np.int8), device=dev).unsqueeze(0)
+            self.occ = row.expand(R, self.N).contiguous()
+            gce.ctx.ce_energy(gce.model, row, self.energy[:1])
+            self.energy[1:] = self.energy[:1]
+        else:
+            self.occ = torch.tensor(np.ascontiguousarray(sig, np.int8), device=dev)
+            gce.ctx.ce_energy(gce.model, self.occ, self.energy)
         self.beta_slot = torch.tensor(np.asarray(betas_slot, np.float64), device=dev)
         self.beta_rep = self.beta_slot.clone()
         self.ladder_id = None if ladder_id is None else torch.tensor(np.asarray(ladder_id, np.int32), device=dev)

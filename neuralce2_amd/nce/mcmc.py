@@ -34,13 +34,20 @@ class NNJobState:
         self.nn = nn
         self.R = R = len(betas)
         sig = np.asarray(sig0)
-        if sig.ndim == 1:
-            sig = np.broadcast_to(sig, (R, nn.N))
-        self.occ = torch.tensor(np.ascontiguousarray(sig, np.int8), device=nn.device)
         self.precision = precision
         self.cache, self.scratch = nn.alloc(R, precision)
         self.energy = torch.empty(R, dtype=torch.float64, device=nn.device)
-        self.anchor()
+        if sig.ndim == 1:
+            # every replica starts from the same configuration: one row uploaded and replicated on the device, its
+            # caches and energy computed once and copied to the other replicas
+            row = torch.tensor(np.ascontiguousarray(sig, np.int8), device=nn.device).unsqueeze(0)
+            self.occ = row.expand(R, nn.N).contiguous()
+            nn.ctx.nn_full_cache(nn.model, row, self.cache[:1], self.energy[:1], precision)
+            self.cache[1:] = self.cache[:1]
+            self.energy[1:] = self.energy[:1]
+        else:
+            self.occ = torch.tensor(np.ascontiguousarray(sig, np.int8), device=nn.device)
+            self.anchor()
         self.beta = torch.tensor(np.asarray(betas, np.float64), device=nn.device)
         self.step = torch.zeros(R, dtype=torch.int64, device=nn.device)
         self.n_accept = torch.zeros(R, dtype=torch.int64, device=nn.device)
