@@ -132,16 +132,22 @@ class PTJobState:
         self.gce = gce
         self.R = R = len(betas_slot)
         self.N = gce.n_atoms
-        sig = np.asarray(sigma0)
-        if sig.shape != (R, self.N) and sig.shape != (self.N,):
-            raise ValueError(f"initial sigma shape {sig.shape} != ({R}, {self.N}) or ({self.N},)")
-        if sig.min() < 0 or sig.max() > 1:
-            raise ValueError("species index out of range for a binary cluster expansion")
         self.energy = torch.empty(R, dtype=torch.float64, device=dev)
-        if sig.ndim == 1:
+        if torch.is_tensor(sigma0):                   # one configuration already on the device (taken as given)
+            if tuple(sigma0.shape) != (self.N,):
+                raise ValueError(f"initial sigma shape {tuple(sigma0.shape)} != ({self.N},)")
+            sig = None
+        else:
+            sig = np.asarray(sigma0)
+            if sig.shape != (R, self.N) and sig.shape != (self.N,):
+                raise ValueError(f"initial sigma shape {sig.shape} != ({R}, {self.N}) or ({self.N},)")
+            if sig.min() < 0 or sig.max() > 1:
+                raise ValueError("species index out of range for a binary cluster expansion")
+        if sig is None or sig.ndim == 1:
             # every replica starts from the same configuration: one row uploaded and replicated on the device, its
             # FP64 energy computed once
-            row = torch.tensor(np.ascontiguousarray(sig, np.int8), device=dev).unsqueeze(0)
+            row = (sigma0.to(device=dev, dtype=torch.int8) if sig is None
+                   else torch.tensor(np.ascontiguousarray(sig, np.int8), device=dev)).unsqueeze(0)
             self.occ = row.expand(R, self.N).contiguous()
             gce.ctx.ce_energy(gce.model, row, self.energy[:1])
             self.energy[1:] = self.energy[:1]

@@ -33,14 +33,15 @@ class NNJobState:
         import torch
         self.nn = nn
         self.R = R = len(betas)
-        sig = np.asarray(sig0)
+        sig = None if torch.is_tensor(sig0) else np.asarray(sig0)   # a device tensor: one configuration
         self.precision = precision
         self.cache, self.scratch = nn.alloc(R, precision)
         self.energy = torch.empty(R, dtype=torch.float64, device=nn.device)
-        if sig.ndim == 1:
+        if sig is None or sig.ndim == 1:
             # every replica starts from the same configuration: one row uploaded and replicated on the device, its
             # caches and energy computed once and copied to the other replicas
-            row = torch.tensor(np.ascontiguousarray(sig, np.int8), device=nn.device).unsqueeze(0)
+            row = (sig0.to(device=nn.device, dtype=torch.int8) if sig is None
+                   else torch.tensor(np.ascontiguousarray(sig, np.int8), device=nn.device)).unsqueeze(0)
             self.occ = row.expand(R, nn.N).contiguous()
             nn.ctx.nn_full_cache(nn.model, row, self.cache[:1], self.energy[:1], precision)
             self.cache[1:] = self.cache[:1]

@@ -6,7 +6,8 @@ continuous batching at chunk granularity:
 
   * admission: a request that has arrived is admitted when its device memory (occupations, caches, counters) fits in
     the budget (default: the smaller of 288 GB and 90 % of the free device memory); requests are taken in arrival
-    order;
+    order.  Its initial occupations are drawn on the device, and the FP64 energy of that one configuration is
+    computed once for all its replicas;
   * one cluster-expansion tick = one chunk of `chunk` steps of every admitted CE job in ONE chunk launch (all
     (job, replica) pairs; the speculative kernel handles jobs of different cells together), the FP64 moment merge
     and the temperature-swap exchange of every CE job, captured once as a HIP graph (nce_pt_cycle) for the current
@@ -78,11 +79,22 @@ def _ladder(spec):
     return temps.astype(np.float32), (1.0 / (KB_CE * temps)).astype(np.float32).astype(np.float64)
 
 
-def _initial_sigma(n_sites, x, seed):
-    """random occupation with round(x N) Pt sites (species index 1), canonical order, from the job seed."""
-    rng = np.random.default_rng(seed)
-    sig = np.zeros(n_sites, np.int8)
-    sig[rng.choice(n_sites, int(round(x * n_sites)), replace=False)] = 1
+def _initial_sigma(n_sites, x, seed, device=None):
+    """random occupation with round(x N) Pt sites (species index 1), canonical order, from the job seed: a numpy
+    array, or with `device` an int8 tensor drawn on that device (a seeded permutation on the GPU, on the current
+    stream, so that even a 10^7-site cell costs the host nothing)."""
+    k = int(round(x * n_sites))
+    if device is None:
+        rng = np.random.default_rng(seed)
+        sig = np.zeros(n_sites, np.int8)
+        sig[rng.choice(n_sites, k, replace=False)] = 1
+        return sig
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    perm = torch.randperm(n_sites, generator=g, device=device)
+    sig = torch.zeros(n_sites, dtype=torch.int8, device=device)
+    sig[perm[:k]] = 1
     return sig
 
 
@@ -92,7 +104,7 @@ class _CEJob:
         self.spec = spec
         self.temps, betas = _ladder(spec)
         self.N = gce.n_atoms
-        self.state = PTJobState(gce, _initial_sigma(self.N, spec.x, spec.seed), betas, seed=spec.seed,
+        self.state = PTJobState(gce, _initial_sigma(self.N, spec.x, spec.seed, gce.device), betas, seed=spec.seed,
                                 sample_interval=chunk, samples_per_chunk=1, job_id=spec.job_id)
         self.steps_done = 0
         self.equilibrating = spec.n_equilib > 0
@@ -138,7 +150,7 @@ class _NCEJob:
         self.temps, betas = _ladder(spec)
         self.betas = (1.0 / (8.617333262e-5 * self.temps.astype(np.float64))).astype(np.float32).astype(np.float64)
         self.N = nn.N
-        self.state = NNJobState(nn, _initial_sigma(self.N, spec.x, spec.seed), self.betas, seed=spec.seed,
+        self.state = NNJobState(nn, _initial_sigma(self.N, spec.x, spec.seed, nn.device), self.betas, seed=spec.seed,
                                 precision=precision, sample_interval=1, samples_per_chunk=chunk)
         self.anchor_every = max(1, int(math.ceil(self.N / chunk)))
         self.n_chunks = 0
