@@ -397,10 +397,13 @@ class Server:
                     if cycle is not None:
                         old_cycles.append(cycle)
                     tile = self._tile_for(active_ce)
+                    # smallest cells first: their replicas take the longest per chunk (window overlaps, near
+                    # pairs), so they are dispatched first and the larger cells' workgroups fill in behind them
+                    order = sorted(active_ce, key=lambda j: j.N)
                     with torch.cuda.stream(self.stream_ce):
                         cycle = self.ctx.pt_cycle(
-                            [j.state.job(accumulate=not j.equilibrating) for j in active_ce], self.chunk, 1,
-                            self.precision, tile, [j.state.exchange_job(None, reorder_last=True) for j in active_ce])
+                            [j.state.job(accumulate=not j.equilibrating) for j in order], self.chunk, 1,
+                            self.precision, tile, [j.state.exchange_job(None, reorder_last=True) for j in order])
                 cycle.launch(1, self.stream_ce)
                 n_ticks += 1
                 ended, rebuild = [], False
