@@ -610,44 +610,63 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     }
                     const int lane = tid & 63, wv = tid >> 6;
                     const int nrt = (prev + 15) >> 4, nct = M >> 4;
-                    constexpr int CT = 4;                // column tiles per pass (A fragment loaded once per pass)
+                    constexpr int CT = 8;                // column tiles per pass (A fragment loaded once per pass)
                     // deeper layers read their input rows from the scratch: eight consecutive channels per lane
                     const bool rows_v = sizeof(Real) == 4 && l > 0 && (Cin & 7) == 0;
+                    auto load_a = [&](int row, int k0, int kb) -> nn_bf16x8 {
+                        nn_bf16x8 av;
+                        if (rows_v) {
+                            NNv4<Real> lo = NNv4<Real>{{(Real)0, (Real)0, (Real)0, (Real)0}}, hi = lo;
+                            if (row < prev && k0 + kb < Cin) {
+                                const NCE_GLOBAL Real *rp = dxg + (int64_t)row * Cin + k0 + kb;
+                                lo = nn_ld4<Real>(rp);
+                                hi = nn_ld4<Real>(rp + 4);
+                            }
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                av[i] = (__bf16)(float)lo.v[i];
+                                av[4 + i] = (__bf16)(float)hi.v[i];
+                            }
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) {
+                                const int c = k0 + kb + i;
+                                float x = 0.0f;
+                                if (row < prev && c < Cin) x = (float)dx(row, c);
+                                av[i] = (__bf16)x;
+                            }
+                        }
+                        return av;
+                    };
                     for (int task = wv; task < nrt * ((nct + CT - 1) / CT); task += NN_BS / 64) {
                         const int rt = task % nrt, cg = (task / nrt) * CT;
                         const int row = rt * 16 + (lane & 15), kb = 8 * (lane >> 4);
                         nn_floatx4 acc[CT];
 #pragma unroll
                         for (int t = 0; t < CT; ++t) acc[t] = nn_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-                        for (int k0 = 0; k0 < Kp; k0 += 32) {
-                            nn_bf16x8 av;
-                            if (rows_v) {
-                                NNv4<Real> lo = NNv4<Real>{{(Real)0, (Real)0, (Real)0, (Real)0}}, hi = lo;
-                                if (row < prev && k0 + kb < Cin) {
-                                    const NCE_GLOBAL Real *rp = dxg + (int64_t)row * Cin + k0 + kb;
-                                    lo = nn_ld4<Real>(rp);
-                                    hi = nn_ld4<Real>(rp + 4);
-                                }
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) {
-                                    av[i] = (__bf16)(float)lo.v[i];
-                                    av[4 + i] = (__bf16)(float)hi.v[i];
-                                }
-                            } else {
-#pragma unroll
-                                for (int i = 0; i < 8; ++i) {
-                                    const int c = k0 + kb + i;
-                                    float x = 0.0f;
-                                    if (row < prev && c < Cin) x = (float)dx(row, c);
-                                    av[i] = (__bf16)x;
-                                }
-                            }
+                        // two 32-wide k blocks per pass: both A fragments are loaded before either is used, so a
+                        // pass waits on the scratch once
+                        for (int k0 = 0; k0 < Kp; k0 += 64) {
+                            const bool two = k0 + 32 < Kp;
+                            const nn_bf16x8 a0 = load_a(row, k0, kb);
+                            nn_bf16x8 a1 = a0;
+                            if (two) a1 = load_a(row, k0 + 32, kb);
 #pragma unroll
                             for (int t = 0; t < CT; ++t) {
                                 if (cg + t < nct) {
                                     const nn_bf16x8 bv =
                                         *(const nn_bf16x8 *)(wt + ((cg + t) * 16 + (lane & 15)) * Kp + k0 + kb);
-                                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+                                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[t], 0, 0, 0);
+                                }
+                            }
+                            if (two) {
+#pragma unroll
+                                for (int t = 0; t < CT; ++t) {
+                                    if (cg + t < nct) {
+                                        const nn_bf16x8 bv =
+                                            *(const nn_bf16x8 *)(wt + ((cg + t) * 16 + (lane & 15)) * Kp + k0 + 32 + kb);
+                                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[t], 0, 0, 0);
+                                    }
                                 }
                             }
                         }
