@@ -208,6 +208,12 @@ typedef struct {
     double de_scale;                   /* model energy change -> total eV (ys * N for the reference models) */
     int32_t ball_cap[8];               /* advisory; the library derives the receptive-field sizes from the window,
                                           the partners and the cell itself */
+    int32_t cache_mode;                /* 0: each replica's cache keeps the standardised layer-0 rows h0s [N][c0]
+                                          (the reference's 'h0' cache mode); 1: it keeps only the hidden layers'
+                                          pre-activations (the reference's 'proj' mode, chosen there for wide layer-0
+                                          models: neuralce/nce/mc_model.py:202-206, local_update_implicit.py:161-183).
+                                          The step never reads h0s rows in either mode; a full recompute in mode 1
+                                          builds them in a temporary buffer, a few replicas at a time */
 } nce_nn_model_desc;
 
 /* One NeuralCE job for nce_nn_run_chunk (device pointers). */

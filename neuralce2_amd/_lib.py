@@ -51,7 +51,7 @@ class NNModelDesc(C.Structure):
                 ("sd", _p), ("lin", _p), ("n_layers", C.c_int32), ("layer_cin", _p), ("layer_d", _p),
                 ("w_self", _p), ("w_shell", _p), ("w_gate", _p), ("b", _p), ("gb", _p), ("activation", C.c_int32),
                 ("head_w", _p), ("head_b", C.c_double), ("readout", C.c_int32), ("de_scale", C.c_double),
-                ("ball_cap", C.c_int32 * 8)]
+                ("ball_cap", C.c_int32 * 8), ("cache_mode", C.c_int32)]
 
 
 class NNJob(C.Structure):
@@ -215,8 +215,9 @@ class Context:
         return Cycle(self, jobs, n_steps, chunks_per_cycle, precision, tile, ex_jobs)
 
     # ── NeuralCE ──
-    def nn_model_create(self, geo, P, mu, sd, act="gate", readout="mean", de_scale=1.0):
-        """upload a NeuralCE model: geometry dict (gpu_layer0_implicit.build_geometry) and expanded parameters."""
+    def nn_model_create(self, geo, P, mu, sd, act="gate", readout="mean", de_scale=1.0, cache_mode="h0"):
+        """upload a NeuralCE model: geometry dict (gpu_layer0_implicit.build_geometry) and expanded parameters;
+        cache_mode 'h0' keeps the layer-0 rows in each replica's cache, 'proj' does not."""
         layers = P["layers"]
         keep = {}
 
@@ -248,6 +249,7 @@ class Context:
         d.head_b = float(P["head"]["b"]); d.readout = {"mean": 0, "sum": 1}[readout]; d.de_scale = float(de_scale)
         for l, cap in enumerate(geo["ball_cap"]):
             d.ball_cap[l] = int(cap)
+        d.cache_mode = {"h0": 0, "proj": 1}[cache_mode]
         mid = C.c_int32()
         check(lib.nce_nn_model_create(self.h, C.byref(d), C.byref(mid)))
         ce, se = C.c_int64(), C.c_int64()

@@ -114,6 +114,7 @@ struct NNModelDev {
     int64_t n_cells, n_sites;
     uint32_t div12_m, div12_s, div2_m, div2_s;
     int32_t W0, J0, P, n_part, C0, L, act, readout, has_gate;
+    int32_t cache_h0;            // 1: the per-replica cache starts with the layer-0 rows h0s [N][C0]
     int32_t cin[NN_MAXL], dd[NN_MAXL], mcat[NN_MAXL];
     int64_t off_self[NN_MAXL], off_shell[NN_MAXL], off_gate[NN_MAXL], off_b[NN_MAXL], off_gb[NN_MAXL];
     int64_t off_wcat[NN_MAXL];   // per layer [cin][mcat]: self | shell_p / n_p | gate_p / n_p, row-major
@@ -194,6 +195,8 @@ struct nce_ctx {
         size_t cap = 0;
     };
     std::vector<Partial> partials;
+    // layer-0 rows of NeuralCE full recomputes for models whose caches do not keep them, one buffer per stream
+    std::vector<Partial> h0_rows;
     std::vector<NNModelDev> nn_models;
     void *d_nn_models = nullptr;
     int cap_nn_models = 0;
@@ -1474,6 +1477,8 @@ extern "C" void nce_ctx_destroy(nce_ctx *c) {
     if (c->d_models) (void)hipFree(c->d_models);
     for (nce_ctx::Partial &pb : c->partials)
         if (pb.p) (void)hipFree(pb.p);
+    for (nce_ctx::Partial &pb : c->h0_rows)
+        if (pb.p) (void)hipFree(pb.p);
     if (c->d_nn_models) (void)hipFree(c->d_nn_models);
     for (nce_ctx::StageRing *ring : c->rings) {
         for (auto &st : ring->stage) {
@@ -1738,13 +1743,13 @@ static int check_model(nce_ctx *c, int32_t model) {
 }
 
 // the partial-sum buffer of stream s, grown to `need` doubles (the old buffer is freed after the stream drains)
-static int partial_for(nce_ctx *c, hipStream_t s, size_t need, double **out) {
+static int stream_buffer(std::vector<nce_ctx::Partial> &bufs, hipStream_t s, size_t need, double **out) {
     nce_ctx::Partial *pb = nullptr;
-    for (nce_ctx::Partial &q : c->partials)
+    for (nce_ctx::Partial &q : bufs)
         if (q.stream == s) pb = &q;
     if (!pb) {
-        c->partials.push_back(nce_ctx::Partial());
-        pb = &c->partials.back();
+        bufs.push_back(nce_ctx::Partial());
+        pb = &bufs.back();
         pb->stream = s;
     }
     if (need > pb->cap) {
@@ -1759,6 +1764,11 @@ static int partial_for(nce_ctx *c, hipStream_t s, size_t need, double **out) {
     }
     *out = pb->p;
     return NCE_OK;
+}
+
+// the stream's buffer of per-block partial sums (doubles), grown as needed
+static int partial_for(nce_ctx *c, hipStream_t s, size_t need, double **out) {
+    return stream_buffer(c->partials, s, need, out);
 }
 
 template <typename K>

@@ -7,7 +7,8 @@
 //   h_l(s)    = tanh(pre_l) * sigmoid(gpre_l)  (gate)  |  tanh(pre_l)  |  pre_l * sigmoid(pre_l)  (SiLU);
 // site energy e(s) = h0s(s) . lin + h_L(s) . w; E = sum_s e / nA + b (mean readout) or sum_s e + b.
 //
-// Per-replica cache (elements of Real): h0s [N][c0] | for each hidden layer: pre [N][d] (| gpre [N][d]).
+// Per-replica cache (elements of Real): h0s [N][c0] (cache mode 0 only) | for each hidden layer: pre [N][d]
+// (| gpre [N][d]).
 //
 // Kernels:
 //   nn_h0_kernel        standardised layer 0 of every site from the layer-0 window and reverse entries
@@ -104,7 +105,7 @@ __host__ __device__ inline int nn_h0_sign_stride(int W0, int C0, int real_bytes)
 template <typename Real>
 __global__ __launch_bounds__(NN_H0_SITES) void nn_h0_kernel(const NNModelDev *__restrict__ models, int model,
                                                             const int8_t *__restrict__ occ_all, int64_t N,
-                                                            Real *__restrict__ cache_all) {
+                                                            Real *__restrict__ h0_all, int64_t h0_stride) {
     extern __shared__ __align__(16) unsigned char smem[];
     const NNModelDev &m = models[model];
     Real *acc = (Real *)smem;                          // [NN_H0_SITES][C0]
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(NN_H0_SITES) void nn_h0_kernel(const NNModelDev *__
             row[m.ent_col[e]] += (Real)(prod * m.ent_coef[e]);
         }
         row[0] = (Real)1;
-        Real *out = cache_all + (int64_t)r * m.cache_elems + site * m.C0;
+        Real *out = h0_all + (int64_t)r * h0_stride + site * m.C0;
         for (int c = 0; c < m.C0; ++c) out[c] = (Real)(((double)row[c] - m.mu[c]) / m.sd[c]);
     }
 }
@@ -148,7 +149,8 @@ constexpr int NN_LDC = 32;               // output channels per thread in the fu
 // of channels, and every output channel sums its terms in the order of the layer formula.
 template <typename Real>
 __global__ __launch_bounds__(256) void nn_layer_kernel(const NNModelDev *__restrict__ models, int model, int l,
-                                                       int64_t N, Real *__restrict__ cache_all) {
+                                                       int64_t N, Real *__restrict__ cache_all,
+                                                       const Real *__restrict__ h0_all, int64_t h0_stride) {
     const NNModelDev &m = models[model];
     const int r = blockIdx.y;
     const int D = m.dd[l], C = m.cin[l];
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void nn_layer_kernel(const NNModelDev *__restr
     if (site >= N) return;
     Real *cache = cache_all + (int64_t)r * m.cache_elems;
     const Real *W = nn_w<Real>(m);
-    const Real *ip = cache + (l == 0 ? 0 : m.off_pre[l - 1]);
+    const Real *ip = l == 0 ? h0_all + (int64_t)r * h0_stride : cache + m.off_pre[l - 1];
     const Real *ig = (l > 0 && m.has_gate) ? cache + m.off_gpre[l - 1] : nullptr;
     auto get = [&](int64_t x, int c) -> Real {
         if (l == 0) return ip[x * C + c];
@@ -209,6 +211,7 @@ __global__ __launch_bounds__(256) void nn_layer_kernel(const NNModelDev *__restr
 template <typename Real>
 __global__ __launch_bounds__(256) void nn_energy_kernel(const NNModelDev *__restrict__ models, int model, int64_t N,
                                                         const Real *__restrict__ cache_all,
+                                                        const Real *__restrict__ h0_all, int64_t h0_stride,
                                                         double *__restrict__ partial) {
     __shared__ double red[4];
     const NNModelDev &m = models[model];
@@ -217,7 +220,7 @@ __global__ __launch_bounds__(256) void nn_energy_kernel(const NNModelDev *__rest
     const Real *cache = cache_all + (int64_t)r * m.cache_elems;
     double e = 0.0;
     if (site < N) {
-        const Real *h0 = cache + site * m.C0;
+        const Real *h0 = h0_all + (int64_t)r * h0_stride + site * m.C0;
         const Real *lin = nn_lin<Real>(m);
         Real acc = (Real)0;
         for (int c = 0; c < m.C0; ++c) acc += h0[c] * lin[c];
@@ -1136,10 +1139,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             const bool accept = eval_dE == nullptr && ((dE_eV <= 0.0) || (u < exp(-beta * dE_eV)));
             if (eval_dE && tid == 0) eval_dE[blockIdx.x] = dE;
             if (accept) {
-                for (int k = tid; k < counts[0] * C0; k += NN_BS) {
-                    const int s = k / C0, c = k - s * C0;
-                    cache[(int64_t)slot_site[s] * C0 + c] += dh0[k];
-                }
+                if (m.cache_h0)
+                    for (int k = tid; k < counts[0] * C0; k += NN_BS) {
+                        const int s = k / C0, c = k - s * C0;
+                        cache[(int64_t)slot_site[s] * C0 + c] += dh0[k];
+                    }
                 for (int l = 0; l < L; ++l) {
                     const int D = m.dd[l];
                     const NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
@@ -1244,6 +1248,8 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     magic_u32((uint32_t)m.d2, &m.div2_m, &m.div2_s);
     m.W0 = d->n_window; m.J0 = d->n_entries; m.P = d->n_shells; m.C0 = d->c0; m.L = d->n_layers;
     m.act = d->activation; m.readout = d->readout; m.has_gate = d->w_gate != nullptr;
+    if (d->cache_mode != 0 && d->cache_mode != 1) return set_err(NCE_ERR_ARG, "cache_mode must be 0 (h0) or 1 (proj)");
+    m.cache_h0 = d->cache_mode == 0 || d->n_layers == 0;
     m.head_b = d->head_b; m.de_scale = d->de_scale;
     int n_part = 0;
     for (int p = 0; p < m.P; ++p) {
@@ -1430,7 +1436,7 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     m.hash_cap = hc;
     // per-replica cache layout; every region starts at a multiple of 4 elements (16-byte rows of four channels)
     auto pad4 = [](int64_t v) { return (v + 3) & ~(int64_t)3; };
-    int64_t o = pad4(N * m.C0);
+    int64_t o = m.cache_h0 ? pad4(N * m.C0) : 0;
     for (int l = 0; l < m.L; ++l) {
         m.off_pre[l] = o; o = pad4(o + N * m.dd[l]);
         m.off_gpre[l] = o; if (m.has_gate) o = pad4(o + N * m.dd[l]);
@@ -1521,21 +1527,45 @@ static int nn_full_cache_impl(nce_ctx *c, int32_t model, const int8_t *occ, int3
     const size_t lds0 = (size_t)NN_H0_SITES * (m.C0 * sizeof(Real) + nn_h0_sign_stride(m.W0, m.C0, (int)sizeof(Real)));
     if (lds0 > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "too many layer-0 channels");
     if (int rc = allow_lds(nn_h0_kernel<Real>, lds0)) return rc;
-    hipLaunchKernelGGL(nn_h0_kernel<Real>, dim3(nb0, R), dim3(NN_H0_SITES), lds0, s, dm, model, occ, N, cache);
-    HIP_TRY(hipGetLastError());
-    for (int l = 0; l < m.L; ++l) {
-        const int64_t nblk = ((N + 255) / 256) * ((m.dd[l] + NN_LDC - 1) / NN_LDC);
-        hipLaunchKernelGGL(nn_layer_kernel<Real>, dim3((unsigned)nblk, R), dim3(256), 0, s, dm, model, l, N, cache);
+    const int nbe = (int)((N + 255) / 256);
+    // layer-0 rows: in the cache (mode 0), else in the stream's temporary buffer, a batch of replicas at a time
+    // (at most 1 GiB of rows, at least one replica)
+    const int64_t row_elems = N * m.C0;
+    int Rb = R;
+    Real *h0 = cache;
+    int64_t h0_stride = m.cache_elems;
+    if (!m.cache_h0) {
+        Rb = (int)std::max<int64_t>(1, std::min<int64_t>(R, ((int64_t)1 << 30) / (row_elems * (int64_t)sizeof(Real))));
+        if (const char *cap = std::getenv("NCE_NN_H0_BATCH"))       // tests: smaller batches on small cells
+            if (std::atoi(cap) > 0) Rb = std::min(Rb, std::atoi(cap));
+        double *buf = nullptr;
+        const size_t need = ((size_t)Rb * row_elems * sizeof(Real) + sizeof(double) - 1) / sizeof(double);
+        if (int rc = stream_buffer(c->h0_rows, s, need, &buf)) return rc;
+        h0 = (Real *)buf;
+        h0_stride = row_elems;
+    }
+    double *partial = nullptr;
+    if (int rc1 = partial_for(c, s, (size_t)Rb * nbe, &partial)) return rc1;
+    for (int r0 = 0; r0 < R; r0 += Rb) {
+        const int nr = std::min(Rb, R - r0);
+        const int8_t *occ_b = occ + (int64_t)r0 * N;
+        Real *cache_b = cache + (int64_t)r0 * m.cache_elems;
+        Real *h0_b = m.cache_h0 ? cache_b : h0;
+        hipLaunchKernelGGL(nn_h0_kernel<Real>, dim3(nb0, nr), dim3(NN_H0_SITES), lds0, s, dm, model, occ_b, N, h0_b,
+                           h0_stride);
+        HIP_TRY(hipGetLastError());
+        for (int l = 0; l < m.L; ++l) {
+            const int64_t nblk = ((N + 255) / 256) * ((m.dd[l] + NN_LDC - 1) / NN_LDC);
+            hipLaunchKernelGGL(nn_layer_kernel<Real>, dim3((unsigned)nblk, nr), dim3(256), 0, s, dm, model, l, N, cache_b,
+                               (const Real *)h0_b, h0_stride);
+            HIP_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(nn_energy_kernel<Real>, dim3(nbe, nr), dim3(256), 0, s, dm, model, N, (const Real *)cache_b,
+                           (const Real *)h0_b, h0_stride, partial);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(nn_energy_finish_kernel, dim3(nr), dim3(64), 0, s, dm, model, partial, nbe, E + r0);
         HIP_TRY(hipGetLastError());
     }
-    const int nbe = (int)((N + 255) / 256);
-    const size_t need = (size_t)R * nbe;
-    double *partial = nullptr;
-    if (int rc1 = partial_for(c, s, need, &partial)) return rc1;
-    hipLaunchKernelGGL(nn_energy_kernel<Real>, dim3(nbe, R), dim3(256), 0, s, dm, model, N, cache, partial);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(nn_energy_finish_kernel, dim3(R), dim3(64), 0, s, dm, model, partial, nbe, E);
-    HIP_TRY(hipGetLastError());
     return NCE_OK;
 }
 

@@ -30,10 +30,25 @@ def silu_network(c0, width, depth, n_shells, seed=0, glorot=True):
     return dict(lin=glorot_u(c0, 1, (c0,)), layers=layers, head=dict(w=glorot_u(cin, 1, (cin,)), b=np.array(0.0)))
 
 
+def default_cache_mode(P):
+    """The reference's choice of cache layout (nce/mc_model.py:202-206): 'h0' (every site's C0 layer-0 channels
+    cached) unless the first layer's projections, (P (x2 with a gate) + 1) x D values per site, are fewer, then
+    'proj'.  Here 'proj' drops the layer-0 rows from the cache: the local update never reads them, since it caches
+    every hidden layer's pre-activations and changes the first by the projected layer-0 change."""
+    if not P["layers"]:
+        return "h0"
+    L1 = P["layers"][0]
+    C0 = len(np.ravel(P["lin"]))
+    D = int(np.shape(L1["self"])[1])
+    n_sh = int(np.shape(L1["shell"])[0])
+    return "h0" if C0 <= (n_sh * (2 if "gate" in L1 else 1) + 1) * D else "proj"
+
+
 class NNModel:
     """A NeuralCE model uploaded for one cell: geometry, parameters, sizes."""
 
-    def __init__(self, cs, atoms, P, mu, sd, shell_edges, act="gate", readout="mean", ys=1.0, ym=0.0, device=0):
+    def __init__(self, cs, atoms, P, mu, sd, shell_edges, act="gate", readout="mean", ys=1.0, ym=0.0, device=0,
+                 cache_mode=None):
         import torch
         from .._lib import Context
         if not torch.cuda.is_available():
@@ -49,8 +64,11 @@ class NNModel:
         self.device = torch.device("cuda", device)
         self.ctx = Context.get(device)
         de_scale = self.ys * self.N if readout == "mean" else 1.0
+        self.cache_mode = cache_mode or default_cache_mode(P)
+        if self.cache_mode not in ("h0", "proj"):
+            raise ValueError(f"cache_mode must be 'h0' or 'proj', not {self.cache_mode!r}")
         self.model, self.cache_elems, self.scratch_elems = self.ctx.nn_model_create(
-            self.geo, P, mu, sd, act=act, readout=readout, de_scale=de_scale)
+            self.geo, P, mu, sd, act=act, readout=readout, de_scale=de_scale, cache_mode=self.cache_mode)
         self.de_scale = de_scale
 
     def alloc(self, R, precision):
@@ -76,17 +94,18 @@ class NNModel:
         return s.contiguous()
 
 
-def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0):
+def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0, cache_mode=None):
     """Build the MC machinery for `atoms` (an ideal-lattice supercell of the model's primitive); see the module doc.
     seed_idx < 0 selects the seed ensemble (mean energy over every saved parameter set), run as one network whose
-    width is the sum of the seeds' widths (model_io.ensemble_params)."""
+    width is the sum of the seeds' widths (model_io.ensemble_params).  cache_mode 'h0' / 'proj' / None (the
+    reference's choice by per-site bytes, default_cache_mode), as setup_implicit's (nce/mc_model.py:175,202-206)."""
     cs = ClusterSpace(mdl["primitive"], list(mdl["cutoffs"]), mdl["chemical_symbols"])
     if seed_idx is not None and int(seed_idx) < 0:
         P, mu, sd = ensemble_params(mdl)
     else:
         P, mu, sd = expand_params(mdl, 0 if seed_idx is None else int(seed_idx))
     nn = NNModel(cs, atoms, P, mu, sd, list(mdl["shell_edges"]), act="gate", readout="mean",
-                 ys=mdl["ys"], ym=mdl["ym"], device=device)
+                 ys=mdl["ys"], ym=mdl["ym"], device=device, cache_mode=cache_mode)
     prec = {"float64": 0, None: 0, "bfloat16": 2, "bf16": 2}.get(dtype, 1)
     geo = nn.geo
 
@@ -150,4 +169,4 @@ def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0):
                 local_delta=local_delta, commit=commit, propose=propose,
                 to_eV_per_atom=lambda E_std: E_std * nn.ys + nn.ym, dE_eV_total=lambda dE: dE * nn.ys * nn.N,
                 implicit=True, geo=geo, perm=geo["perm"], inv_perm=geo["inv_perm"], nn=nn, dtype=dtype,
-                precision=prec, cache_mode="h0")
+                precision=prec, cache_mode=nn.cache_mode)

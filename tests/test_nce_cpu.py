@@ -244,3 +244,28 @@ def test_unsupported_parameter_sets_are_refused(model):
     L2 = {k: v for k, v in P["layers"][0].items() if k not in ("gate", "gb")}
     with pytest.raises(NotImplementedError, match="gate"):
         check_supported(dict(P, layers=[P["layers"][0], L2]))
+
+
+def test_default_cache_mode_follows_the_reference_rule(model):
+    """the cache layout is chosen as the reference's setup_implicit does (nce/mc_model.py:202-206): 'h0' while the
+    layer-0 width C0 is at most the first layer's projection width (P shells, x2 with a gate, + 1) x D, else
+    'proj'; no hidden layer: 'h0'.  Shipped PtNi model: C0 = 18 <= (3 x 2 + 1) x 16 = 112; the 3 x 64 SiLU
+    network: 41 <= 2 x 64; a Fe-Ni-Cr-like width of 131 channels with the PtNi layer shapes: 131 > 112."""
+    from neuralce2_amd.nce.mc_model import default_cache_mode
+    from neuralce2_amd.nce.model_io import expand_params
+    mdl, _ = model
+    P, _, _ = expand_params(mdl, 0)
+    assert len(np.ravel(P["lin"])) == 18 and default_cache_mode(P) == "h0"
+    cs, _, Ps = _silu_setup()
+    assert default_cache_mode(Ps) == "h0"
+    rng = np.random.default_rng(0)
+    L1 = dict(P["layers"][0])
+    for k in ("self", "shell", "gate"):
+        shp = list(np.shape(L1[k]))
+        shp[-2] = 131
+        L1[k] = rng.normal(size=shp)
+    wide = dict(P, lin=rng.normal(size=131), layers=[L1])
+    assert default_cache_mode(wide) == "proj"
+    narrow = dict(wide, lin=rng.normal(size=112))
+    assert default_cache_mode(narrow) == "h0"
+    assert default_cache_mode(dict(P, layers=[])) == "h0"
