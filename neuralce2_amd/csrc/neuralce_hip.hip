@@ -146,9 +146,27 @@ struct NNModelDev {
     int64_t pre_list_off[NN_MAXL], pre_end_off[NN_MAXL];
     // per precision: which per-step arrays live in LDS (else in the scratch): 0 none, 1 the layer-0 change of
     // ball 0, 2 and the sorted partner hits, 3 and the projections
-    int32_t lds_mode32, lds_mode64;
+    int32_t lds_mode32, lds_mode64, lds_mode16;
     int64_t pr_elems, hit_elems, w_elems;   // largest projection block / hit list / used projection matrix
-    int32_t mfma_ok;             // every layer's projection width is a multiple of 16 (BF16 MFMA tiles)
+    int32_t mfma_ok;             // every layer's projection width is a multiple of 16 (FP32 MFMA tiles)
+    // BF16 layers (gathered shell sums times the stacked matrix on MFMA): every width D a multiple of 16,
+    // D (1 + gate) <= 128 and every input width <= 64.  Per layer the input width padded to 32 (cpad), the stacked
+    // matrix's k extent
+    // cpad (1 + 2P) (self, then each shell's sum as a bf16 high and low part) and its LDS row stride (+8).
+    int32_t fused_ok, xb_stride, xb_zrow;   // xb_zrow: the all-zero row of either input-row buffer
+    int32_t cpad[NN_MAXL], kf[NN_MAXL];
+    int64_t wf_bytes;            // largest stacked matrix in LDS (bf16)
+    const uint16_t *wf_img;      // every layer's stacked matrix as bf16 bits, in its LDS layout (row stride kf + 8)
+    int64_t wf_off[NN_MAXL];
+    // BF16 layers: per target a row of nbr_w = P * nbr_ps uint16 source slots, shell p's partners in columns
+    // p * nbr_ps + (index within the shell), the zero row xb_zrow where the partner is outside ball_{l-1}.
+    // Fixed balls: one table per layer in device memory (pre_nbr); hash steps build theirs in the scratch.
+    int32_t nbr_w, nbr_ps;
+    const int32_t *part_col;     // [n_part] column of each partner in a neighbour-table row
+    const uint16_t *pre_nbr;
+    int64_t pre_nbr_off[NN_MAXL], soff_nbr;
+    int64_t xb_off1;             // the BF16 layers' two input-row buffers (bf16 [rows][xb_stride]) start at the
+                                 // projection block; the second this many bf16 elements after the first
     int64_t off_pre[NN_MAXL], off_gpre[NN_MAXL], cache_elems;
     int64_t soff_dh0, soff_pr, soff_tgt, soff_list, soff_dpre[NN_MAXL], soff_dgpre[NN_MAXL], soff_dh[2], scratch_elems;
 };

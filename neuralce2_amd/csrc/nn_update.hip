@@ -58,6 +58,13 @@ __device__ inline Real nn_act(const NNModelDev &m, Real pre, Real gpre) {
     if (m.has_gate) out *= nn_sigmoid(gpre);
     return out;
 }
+// the same with the kind fixed at compile time: 0 tanh, 1 SiLU, 2 gated tanh
+template <int ACT, typename Real>
+__device__ inline Real nn_act_k(Real pre, Real gpre) {
+    if constexpr (ACT == 1) return pre * nn_sigmoid(pre);
+    else if constexpr (ACT == 2) return nn_tanh(pre) * nn_sigmoid(gpre);
+    else return nn_tanh(pre);
+}
 
 __device__ inline int64_t nn_site(const NNModelDev &m, int64_t center, int4 e) {
     const uint32_t c = (uint32_t)center;
@@ -275,7 +282,7 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
     L.misc = o;      o = align16(o + (4 + NN_MAXL + 1) * sizeof(int));
     L.la = o;        o = align16(o + NN_LA * sizeof(NNProp));
     L.wv = o;        o = align16(o + (size_t)win_n * 3 * sizeof(int));
-    L.part = o;      o = align16(o + (size_t)n_part * (sizeof(int4) + sizeof(int)) + (size_t)n_c0 * sizeof(int));
+    L.part = o;      o = align16(o + (size_t)n_part * (sizeof(int4) + 2 * sizeof(int)) + (size_t)n_c0 * sizeof(int));
     L.w = o;         o = align16(o + w_bytes);
     L.dh0 = o;       o = align16(o + dh0_bytes);
     L.list = o;      o = align16(o + list_bytes);
@@ -283,12 +290,15 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
     L.total = o;
     return L;
 }
+// fused: the BF16 layers, whose stacked matrix is staged instead of the projection matrix and which keep neither a
+// hit list nor a projection block in LDS
 template <typename Real>
-__host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode) {
-    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, m.n_part, m.c0_eff, (size_t)m.w_elems * sizeof(Real),
+__host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode, bool fused = false) {
+    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, m.n_part, m.c0_eff,
+                         fused ? (size_t)m.wf_bytes : (size_t)m.w_elems * sizeof(Real),
                          mode >= 1 ? (size_t)m.ball_cap[0] * m.C0 * sizeof(Real) : 0,
-                         mode >= 2 ? (size_t)m.hit_elems * sizeof(int) : 0,
-                         mode >= 3 ? (size_t)m.pr_elems * sizeof(Real) : 0);
+                         mode >= 2 && !fused ? (size_t)m.hit_elems * sizeof(int) : 0,
+                         mode >= 3 && !fused ? (size_t)m.pr_elems * sizeof(Real) : 0);
 }
 
 // multiplicative hashing: the position is the top log2(cap) bits of the 32-bit product (cap = mask + 1 >= 64, a power
@@ -407,7 +417,11 @@ __device__ inline void nn_st4(NCE_GLOBAL Real *p, const NNv4<Real> &v) {
     *(NCE_GLOBAL vec *)p = vec{v.v[0], v.v[1], v.v[2], v.v[3]};
 }
 typedef __bf16 nn_bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 nn_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float nn_floatx4 __attribute__((ext_vector_type(4)));
+constexpr int NN_FCT = 4;                // BF16 layers: output tiles of 16 columns per pass over the hits
+constexpr int NN_FCH = 4;                // BF16 layers: groups of four neighbour-table columns loaded together
+constexpr int NN_FMAXT = 8;              // BF16 layers: at most 8 output tiles (D (1 + gate) <= 128)
 
 template <typename Real, int MODE, bool MFMA>
 __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__restrict__ models,
@@ -422,7 +436,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     const int tid = threadIdx.x;
     const int L = m.L, C0 = m.C0, n_part = m.n_part, P = m.P;
     const int mask = m.hash_cap - 1;
-    const NNLds lay = nn_lds_for<Real>(m, MODE);
+    const NNLds lay = nn_lds_for<Real>(m, MODE, MFMA);
     int *keys = (int *)(smem + lay.keys);
     int *vals = (int *)(smem + lay.vals);
     int *slot_site = (int *)(smem + lay.slot_site);
@@ -434,14 +448,25 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     // shell partners (offset and shell of each) and the used layer-0 channels, staged once per launch
     int4 *s_part = (int4 *)(smem + lay.part);
     int *s_part_shell = (int *)(s_part + m.n_part);
-    int *s_c0 = s_part_shell + m.n_part;
+    int *s_part_col = s_part_shell + m.n_part;
+    int *s_c0 = s_part_col + m.n_part;
     for (int k = tid; k < m.n_part; k += NN_BS) {
         s_part[k] = m.part[k];
         s_part_shell[k] = m.part_shell[k];
+        s_part_col[k] = m.part_col[k];
     }
     for (int k = tid; k < m.c0_eff; k += NN_BS) s_c0[k] = m.c0_ch[k];
     for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
     if (tid == 0) misc[0] = 0;
+    if constexpr (MFMA) {
+        // the zero row of both input-row buffers of the BF16 layers (read for absent hits and self terms)
+        NCE_GLOBAL __bf16 *xb = (NCE_GLOBAL __bf16 *)(gptr((Real *)job.scratch) + (int64_t)r * m.scratch_elems +
+                                                      m.soff_pr);
+        const int n8 = m.xb_stride >> 3;
+        if (tid < 2 * n8)
+            *(NCE_GLOBAL nn_bf16x8 *)(xb + (tid < n8 ? 0 : m.xb_off1) + (int64_t)m.xb_zrow * m.xb_stride +
+                                      8 * (tid < n8 ? tid : tid - n8)) = nn_bf16x8{};
+    }
     __syncthreads();
 
     // buffers in device memory are addressed as global (flat accesses would also count against lgkmcnt, so every
@@ -587,6 +612,21 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     c -= c >= C0 ? C0 : 0;
                 }
             }
+            if constexpr (MFMA) {
+                // the first layer's input rows: the used channels of the layer-0 change as bf16, zero up to cpad[0]
+                const int n8 = m.cpad[0] >> 3;
+                NCE_GLOBAL __bf16 *x0 = (NCE_GLOBAL __bf16 *)(scr + m.soff_pr);
+                for (int k = tid; k < counts[0] * n8; k += NN_BS) {
+                    const int s = k / n8, c0 = (k - s * n8) * 8;
+                    nn_bf16x8 v;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int c = c0 + i;
+                        v[i] = (__bf16)(c < m.c0_eff ? (float)dh0[(int64_t)s * C0 + s_c0[c]] : 0.0f);
+                    }
+                    *(NCE_GLOBAL nn_bf16x8 *)(x0 + (int64_t)s * m.xb_stride + c0) = v;
+                }
+            }
             // hidden layers: projection of the previous layer's change, partner hits sorted by target, gather
             for (int l = 0; l < L; ++l) {
                 const int Cin = l == 0 ? m.c0_eff : m.cin[l], D = m.dd[l], M = m.mcat[l];
@@ -599,89 +639,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const NCE_GLOBAL Real *Wc = W + m.off_wcat[l];
                 // (1) pr[q][j] = dx[q] . Wcat[:, j]
                 if constexpr (MFMA) {
-                    // BF16 MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulation): the layer's matrix staged in LDS as
-                    // bf16, transposed [j][k] with k padded to 32; one 16 x 16 output tile per wave at a time.
-                    // Lane l holds A[row l%16][k 8(l/16)..+7], B[k 8(l/16)..+7][col l%16], C[rows 4(l/16)..+3][col l%16].
-                    const int Kp = (Cin + 31) & ~31;
-                    __bf16 *wt = (__bf16 *)wl;
+                    // BF16 layers: the stacked matrix (layout at nn_stack_bf16), built at model upload and copied
+                    // 16 bytes per thread
                     if (!w_staged) {
-                        for (int k = tid; k < M * Kp; k += NN_BS) {
-                            const int j = k / Kp, c = k - j * Kp;
-                            wt[k] = c < Cin ? (__bf16)Wc[(int64_t)(l == 0 ? s_c0[c] : c) * M + j] : (__bf16)0.0f;
-                        }
-                        __syncthreads();
-                    }
-                    const int lane = tid & 63, wv = tid >> 6;
-                    const int nrt = (prev + 15) >> 4, nct = M >> 4;
-                    constexpr int CT = 8;                // column tiles per pass (A fragment loaded once per pass)
-                    // deeper layers read their input rows from the scratch: eight consecutive channels per lane
-                    const bool rows_v = sizeof(Real) == 4 && l > 0 && (Cin & 7) == 0;
-                    auto load_a = [&](int row, int k0, int kb) -> nn_bf16x8 {
-                        nn_bf16x8 av;
-                        if (rows_v) {
-                            NNv4<Real> lo = NNv4<Real>{{(Real)0, (Real)0, (Real)0, (Real)0}}, hi = lo;
-                            if (row < prev && k0 + kb < Cin) {
-                                const NCE_GLOBAL Real *rp = dxg + (int64_t)row * Cin + k0 + kb;
-                                lo = nn_ld4<Real>(rp);
-                                hi = nn_ld4<Real>(rp + 4);
-                            }
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                av[i] = (__bf16)(float)lo.v[i];
-                                av[4 + i] = (__bf16)(float)hi.v[i];
-                            }
-                        } else {
-#pragma unroll
-                            for (int i = 0; i < 8; ++i) {
-                                const int c = k0 + kb + i;
-                                float x = 0.0f;
-                                if (row < prev && c < Cin) x = (float)dx(row, c);
-                                av[i] = (__bf16)x;
-                            }
-                        }
-                        return av;
-                    };
-                    for (int task = wv; task < nrt * ((nct + CT - 1) / CT); task += NN_BS / 64) {
-                        const int rt = task % nrt, cg = (task / nrt) * CT;
-                        const int row = rt * 16 + (lane & 15), kb = 8 * (lane >> 4);
-                        nn_floatx4 acc[CT];
-#pragma unroll
-                        for (int t = 0; t < CT; ++t) acc[t] = nn_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-                        // two 32-wide k blocks per pass: both A fragments are loaded before either is used, so a
-                        // pass waits on the scratch once
-                        for (int k0 = 0; k0 < Kp; k0 += 64) {
-                            const bool two = k0 + 32 < Kp;
-                            const nn_bf16x8 a0 = load_a(row, k0, kb);
-                            nn_bf16x8 a1 = a0;
-                            if (two) a1 = load_a(row, k0 + 32, kb);
-#pragma unroll
-                            for (int t = 0; t < CT; ++t) {
-                                if (cg + t < nct) {
-                                    const nn_bf16x8 bv =
-                                        *(const nn_bf16x8 *)(wt + ((cg + t) * 16 + (lane & 15)) * Kp + k0 + kb);
-                                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[t], 0, 0, 0);
-                                }
-                            }
-                            if (two) {
-#pragma unroll
-                                for (int t = 0; t < CT; ++t) {
-                                    if (cg + t < nct) {
-                                        const nn_bf16x8 bv =
-                                            *(const nn_bf16x8 *)(wt + ((cg + t) * 16 + (lane & 15)) * Kp + k0 + 32 + kb);
-                                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[t], 0, 0, 0);
-                                    }
-                                }
-                            }
-                        }
-#pragma unroll
-                        for (int t = 0; t < CT; ++t) {
-                            if (cg + t >= nct) continue;
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                const int r0 = rt * 16 + 4 * (lane >> 4) + i;
-                                if (r0 < prev) pr[(int64_t)r0 * M + (cg + t) * 16 + (lane & 15)] = (Real)acc[t][i];
-                            }
-                        }
+                        const int n16 = (D * (m.has_gate ? 2 : 1) * (m.kf[l] + 8)) >> 3;
+                        typedef int i4 __attribute__((ext_vector_type(4)));
+                        const NCE_GLOBAL i4 *src = (const NCE_GLOBAL i4 *)(gptr(m.wf_img) + m.wf_off[l]);
+                        for (int k = tid; k < n16; k += NN_BS) ((i4 *)wl)[k] = src[k];
                     }
                 } else if (sizeof(Real) == 4 && m.mfma_ok) {
                     // FP32 MFMA (v_mfma_f32_16x16x4_f32): the layer's matrix staged in LDS as [c][j]; one 16 x 16
@@ -816,7 +780,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 if (fixed) {
                     nb = 2 * m.pre_n[l + 1];
                     counts[l + 1] = nb;
-                    if (!pre_in_list) {
+                    if (!pre_in_list && !MFMA) {        // (the BF16 layers read the neighbour tables in place)
                         const NCE_GLOBAL int *hl = gptr(m.pre_list) + m.pre_list_off[l];
                         const NCE_GLOBAL int *he = gptr(m.pre_end) + m.pre_end_off[l];
                         for (int i = tid; i < nh; i += NN_BS) list[i] = hl[i];
@@ -846,6 +810,21 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 __syncthreads();
                 nb = misc[0];
                 counts[l + 1] = nb;
+                if constexpr (MFMA) {
+                    // BF16 layers: the neighbour table of ball_l (every column the zero row, then each hit's source
+                    // in its partner's column; a target and a partner have at most one source)
+                    NCE_GLOBAL uint32_t *nb32 = (NCE_GLOBAL uint32_t *)(scr + m.soff_nbr);
+                    const uint32_t z2 = (uint32_t)m.xb_zrow * 0x10001u;
+                    for (int e = tid; e < (nb * m.nbr_w) >> 1; e += NN_BS) nb32[e] = z2;
+                    __syncthreads();
+                    NCE_GLOBAL uint16_t *nbt = (NCE_GLOBAL uint16_t *)(scr + m.soff_nbr);
+                    q = q_0; kk = kk_0;
+                    for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
+                        if (kk >= n_part) { kk -= n_part; ++q; }
+                        nbt[(int64_t)vals[tgt[i]] * m.nbr_w + s_part_col[kk]] = (uint16_t)q;
+                    }
+                    __syncthreads();
+                } else {
                 for (int s = tid; s < nb; s += NN_BS) cnt[s] = 0;
                 __syncthreads();
                 for (int i = tid; i < nh; i += NN_BS) {
@@ -864,6 +843,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 }
                 __syncthreads();
                 }
+                }
                 // (4) change of the pre-activations, new activations, change of the layer output
                 NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
                 NCE_GLOBAL Real *dgpre = scr + m.soff_dgpre[l];
@@ -873,7 +853,180 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const bool last = l + 1 == L;
                 const int og = D * (1 + P);
                 constexpr int U = 4;                     // items per thread in flight together
-                if (sizeof(Real) == 4 && MODE < 3 && (D & 7) == 0) {
+                if constexpr (MFMA) {
+                // BF16 layers, one wave per 16 targets (lane: target tile*16 + lane%16, channel group lane/16):
+                // the change of the pre-activations of target t is C = Wstack^T . [x_t | sum_p hi | sum_p lo], a
+                // 16-column tile of MFMA v_mfma_f32_16x16x32_bf16 per 16 output columns.  The B operand is built
+                // in registers: x_t is the input row (bf16, the zero row for t outside ball_{l-1}); sum_p is the
+                // FP32 sum of the input rows of t's hits in shell p, split into bf16 high and low parts (their sum
+                // equals it to 2^-16 relative).  A lane holds C[columns 4 g .. + 3 of each tile][target], so the
+                // cache rows, the scratch rows and the next layer's input rows move four channels at a time.
+                // The hits of a target come from its neighbour-table row (fixed width, the zero row where a
+                // partner lies outside ball_{l-1}), so every lane loads the same number of rows, with no branches.
+                const int Ks = m.kf[l] + 8;
+                const int nct = (D * (m.has_gate ? 2 : 1)) >> 4;
+                const __bf16 *wt = (const __bf16 *)wl;
+                const uint32_t xrow = 2u * (uint32_t)m.xb_stride;   // bytes per input row
+                const int zrow = m.xb_zrow;
+                const NCE_GLOBAL char *xin =
+                    (const NCE_GLOBAL char *)((const NCE_GLOBAL __bf16 *)(scr + m.soff_pr) + (l & 1) * m.xb_off1);
+                NCE_GLOBAL __bf16 *xout = (NCE_GLOBAL __bf16 *)(scr + m.soff_pr) + ((l + 1) & 1) * m.xb_off1;
+                const NCE_GLOBAL uint16_t *nbt = fixed ? gptr(m.pre_nbr) + m.pre_nbr_off[l]
+                                                       : (const NCE_GLOBAL uint16_t *)(scr + m.soff_nbr);
+                const int nbw = m.nbr_w, nps = m.nbr_ps;
+                const int lane = tid & 63, g = lane >> 4, tl = lane & 15;
+                const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+                // (tiles of NN_FCT output tiles each: a gated model of D = 64 passes over the hits twice)
+                const int ngrp = (nct + NN_FCT - 1) / NN_FCT;
+                const int ntask = ((nb + 15) >> 4) * ngrp;
+                // NSB 32-wide k steps per segment (input width 32 NSB): in step s lane group g takes channels
+                // 32 s + 8 g .. +7, so the four lane groups of a target read 64 contiguous bytes of each row
+                auto layer = [&]<int NSB>() {
+                    constexpr int NZ = 8 * NSB;
+                    const uint32_t cob = 16u * (uint32_t)g;   // byte offset of the lane's channels in step 0
+                    for (int task = wave; task < ntask; task += NN_BS / 64) {
+                        const int tile = task / ngrp, c0t = (task - tile * ngrp) * NN_FCT;
+                        const int t = tile * 16 + tl;
+                        const bool tv = t < nb;
+                        // A fragment rows of the group's output tiles (a tile past the last reads the last: its
+                        // products are computed and dropped, so no MFMA waits on a branch)
+                        const __bf16 *wrow[NN_FCT];
+#pragma unroll
+                        for (int ct = 0; ct < NN_FCT; ++ct)
+                            wrow[ct] = wt + (min(c0t + ct, nct - 1) * 16 + tl) * Ks + 8 * g;
+                        nn_floatx4 acc[NN_FCT];
+#pragma unroll
+                        for (int ct = 0; ct < NN_FCT; ++ct) acc[ct] = nn_floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                        auto mma = [&](int k, const nn_bf16x8 &bv) {
+                            nn_bf16x8 av[NN_FCT];
+#pragma unroll
+                            for (int ct = 0; ct < NN_FCT; ++ct) av[ct] = *(const nn_bf16x8 *)(wrow[ct] + k);
+#pragma unroll
+                            for (int ct = 0; ct < NN_FCT; ++ct)
+                                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[ct], bv, acc[ct], 0, 0, 0);
+                        };
+                        auto ldrow = [&](int row, int half) -> nn_bf16x8 {
+                            return *(const NCE_GLOBAL nn_bf16x8 *)(xin + (uint32_t)row * xrow + cob + 64u * half);
+                        };
+                        {
+                            const int xr = t < prev ? t : zrow;
+#pragma unroll
+                            for (int s = 0; s < NSB; ++s) mma(32 * s, ldrow(xr, s));
+                        }
+                        // the selectors of the bf16 sums live in registers: the compiler encodes the pair (1, 0) as
+                        // the inline constant 1.0, which the hardware reads as the 32-bit float 1.0, i.e. (0, 1)
+                        typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+                        bf2 lo1 = bf2{(__bf16)1.0f, (__bf16)0.0f}, hi1 = bf2{(__bf16)0.0f, (__bf16)1.0f};
+                        asm volatile("" : "+v"(lo1));
+                        asm volatile("" : "+v"(hi1));
+                        // the target's neighbour-table row (target 0's for a lane past the last target)
+                        const NCE_GLOBAL uint16_t *tb = nbt + (int64_t)(tv ? t : 0) * nbw;
+                        for (int p = 0; p < P; ++p) {
+                            float z[NZ];
+#pragma unroll
+                            for (int e = 0; e < NZ; ++e) z[e] = 0.0f;
+                            // a row's channels added with v_dot2_f32_bf16 against (1, 0) and (0, 1): one instruction
+                            // per channel, exact products, one FP32 rounding per addition (equal to the FP32 sum)
+                            auto add = [&](const nn_bf16x8 &v, int o) {
+#pragma unroll
+                                for (int e = 0; e < 4; ++e) {
+                                    const bf2 pr2 = bf2{v[2 * e], v[2 * e + 1]};
+                                    z[o + 2 * e] = __builtin_amdgcn_fdot2_f32_bf16(pr2, lo1, z[o + 2 * e], false);
+                                    z[o + 2 * e + 1] = __builtin_amdgcn_fdot2_f32_bf16(pr2, hi1, z[o + 2 * e + 1], false);
+                                }
+                            };
+                            // shell p's columns in groups of four (one 8-byte table load each): NN_FCH groups' table
+                            // entries load together, then each group's four rows (columns without a partner in
+                            // ball_{l-1} hold the zero row)
+                            typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+                            const NCE_GLOBAL u2 *tp = (const NCE_GLOBAL u2 *)(tb + p * nps);
+                            const int nch = nps >> 2;
+                            for (int c0 = 0; c0 < nch; c0 += NN_FCH) {
+                                u2 e4[NN_FCH];
+#pragma unroll
+                                for (int cc = 0; cc < NN_FCH; ++cc) e4[cc] = tp[min(c0 + cc, nch - 1)];
+#pragma unroll
+                                for (int cc = 0; cc < NN_FCH; ++cc) {
+                                    if (c0 + cc >= nch) break;
+                                    const int rw[4] = {(int)(e4[cc].x & 0xffffu), (int)(e4[cc].x >> 16),
+                                                       (int)(e4[cc].y & 0xffffu), (int)(e4[cc].y >> 16)};
+                                    nn_bf16x8 v[4][NSB];
+#pragma unroll
+                                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                                        for (int s = 0; s < NSB; ++s) v[u][s] = ldrow(rw[u], s);
+#pragma unroll
+                                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                                        for (int s = 0; s < NSB; ++s) add(v[u][s], 8 * s);
+                                }
+                            }
+#pragma unroll
+                            for (int s = 0; s < NSB; ++s) {
+                                nn_bf16x8 hi, lo;
+#pragma unroll
+                                for (int e = 0; e < 8; ++e) {
+                                    const float x = z[8 * s + e];
+                                    hi[e] = (__bf16)x;
+                                    lo[e] = (__bf16)(x - (float)hi[e]);
+                                }
+                                const int k = (1 + 2 * p) * 32 * NSB + 32 * s;
+                                mma(k, hi);
+                                mma(k + 32 * NSB, lo);
+                            }
+                        }
+                        if (tv) {
+                            using V4 = NNv4<Real>;
+                            const V4 z4 = V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
+                            const int64_t so = (int64_t)slot_site[t] * D;
+                            // four output channels jc .. jc + 3 of target t (ACT: the activation, fixed per tile)
+                            auto out = [&]<int ACT>(int jc, const nn_floatx4 &ap, const nn_floatx4 *ag) {
+                                const V4 pv = nn_ld4<Real>(cpre + so + jc);
+                                const V4 dp = V4{{ap[0], ap[1], ap[2], ap[3]}};
+                                V4 gv = z4, dg = z4, hv;
+                                if (ag) {
+                                    gv = nn_ld4<Real>(cgpre + so + jc);
+                                    dg = V4{{(*ag)[0], (*ag)[1], (*ag)[2], (*ag)[3]}};
+                                }
+#pragma unroll
+                                for (int e = 0; e < 4; ++e)
+                                    hv.v[e] = nn_act_k<ACT, Real>(pv.v[e] + dp.v[e], gv.v[e] + dg.v[e]) - nn_act_k<ACT, Real>(pv.v[e], gv.v[e]);
+                                nn_st4<Real>(dpre + (int64_t)t * D + jc, dp);
+                                if (ag) nn_st4<Real>(dgpre + (int64_t)t * D + jc, dg);
+                                if (last) {
+#pragma unroll
+                                    for (int e = 0; e < 4; ++e) part += (double)(hv.v[e] * hw[jc + e]);
+                                } else {
+                                    nn_bf16x4 hb;
+#pragma unroll
+                                    for (int e = 0; e < 4; ++e) hb[e] = (__bf16)(float)hv.v[e];
+                                    *(NCE_GLOBAL nn_bf16x4 *)(xout + (int64_t)t * m.xb_stride + jc) = hb;
+                                }
+                            };
+                            if (m.has_gate) {
+                                // gated models alternate pre-activation and gate tiles (tanh times the gate's sigmoid)
+#pragma unroll
+                                for (int ct = 0; ct < NN_FCT / 2; ++ct)
+                                    if (c0t + 2 * ct < nct)
+                                        out.template operator()<2>((c0t / 2 + ct) * 16 + 4 * g, acc[2 * ct], &acc[2 * ct + 1]);
+                            } else if (m.act == 1) {
+#pragma unroll
+                                for (int ct = 0; ct < NN_FCT; ++ct)
+                                    if (c0t + ct < nct) out.template operator()<1>((c0t + ct) * 16 + 4 * g, acc[ct], nullptr);
+                            } else {
+#pragma unroll
+                                for (int ct = 0; ct < NN_FCT; ++ct)
+                                    if (c0t + ct < nct) out.template operator()<0>((c0t + ct) * 16 + 4 * g, acc[ct], nullptr);
+                            }
+                            // a width that is not a multiple of 32: the next layer's padding channels D .. D + 15 are 0
+                            if (!last && (D & 31) && c0t == 0)
+                                *(NCE_GLOBAL nn_bf16x4 *)(xout + (int64_t)t * m.xb_stride + D + 4 * g) = nn_bf16x4{};
+                        }
+                    }
+                };
+                if (m.cpad[l] == 64) layer.template operator()<2>();
+                else layer.template operator()<1>();
+                } else if (sizeof(Real) == 4 && MODE < 3 && (D & 7) == 0) {
                 // projections in the scratch: eight channels per item (FP32), two 16-byte halves, so each hit's list
                 // entry is read from memory once for both (with the projections in LDS four channels per item and
                 // more hits in flight do better)
@@ -1223,11 +1376,51 @@ static int nn_sync_models(nce_ctx *c) {
 
 // LDS placement of the chunk kernel's per-step arrays: the most that fits beside the hash table, in the order
 // layer-0 change, partner hits, projections
+// (the BF16 layers keep only the layer-0 change there: at most mode 1)
 template <typename Real>
-static int nn_lds_mode(const NNModelDev &m) {
-    int mode = 3;
-    while (mode > 0 && nn_lds_for<Real>(m, mode).total > 160 * 1024) --mode;
+static int nn_lds_mode(const NNModelDev &m, bool fused = false) {
+    int mode = fused ? 1 : 3;
+    while (mode > 0 && nn_lds_for<Real>(m, mode, fused).total > 160 * 1024) --mode;
     return mode;
+}
+
+// float -> bf16 bits, round to nearest even (the device conversion v_cvt_pk_bf16_f32 rounds the same way)
+static uint16_t nn_bf16_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// The stacked matrix of BF16 layer l in its LDS layout: one row per output column j (gated models alternate 16-column
+// tiles of pre-activations and gates), Kf + 8 elements per row, k = segment * cpad + c for input channel c of the
+// segments self, then shell p's sum as a high (1 + 2p) and a low (2 + 2p) part.  Wc: the layer's [cin][mcat] matrix
+// in float.
+static void nn_stack_bf16(const NNModelDev &m, int l, const float *Wc, const int32_t *c0_ch, std::vector<uint16_t> &img) {
+    const int D = m.dd[l], M = m.mcat[l], P = m.P, Cp = m.cpad[l], Kf = m.kf[l], Ks = Kf + 8;
+    const int Cin = l == 0 ? m.c0_eff : m.cin[l];
+    const int Nout = D * (m.has_gate ? 2 : 1);
+    for (int j = 0; j < Nout; ++j)
+        for (int kk = 0; kk < Ks; ++kk) {
+            float v = 0.0f;
+            if (kk < Kf) {
+                const int seg = kk / Cp, c = kk - seg * Cp;
+                const int T = j >> 4;
+                const bool isg = m.has_gate && (T & 1);
+                const int dch = m.has_gate ? (T >> 1) * 16 + (j & 15) : j;
+                if (c < Cin) {
+                    const int64_t row = (int64_t)(l == 0 ? c0_ch[c] : c) * M;
+                    if (seg == 0) {
+                        if (!isg) v = Wc[row + dch];
+                    } else {
+                        const int p = (seg - 1) >> 1;
+                        v = Wc[row + (isg ? D * (1 + P + p) : D * (1 + p)) + dch];
+                    }
+                }
+            }
+            img.push_back(nn_bf16_bits(v));
+        }
 }
 
 extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32_t *model_id) {
@@ -1327,9 +1520,16 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
         return true;
     };
     if (!chk4(d->window, m.W0) || !chk4(d->partners, n_part)) return set_err(NCE_ERR_ARG, "offsets not reduced");
-    std::vector<int32_t> part_shell(std::max(n_part, 1));
+    std::vector<int32_t> part_shell(std::max(n_part, 1)), part_col(std::max(n_part, 1));
+    int max_sc = 1;
+    for (int p = 0; p < m.P; ++p) max_sc = std::max(max_sc, (int)d->shell_count[p]);
+    m.nbr_ps = (max_sc + 3) & ~3;
+    m.nbr_w = std::max(1, m.P) * m.nbr_ps;
     for (int p = 0, k = 0; p < m.P; ++p)
-        for (int q = 0; q < d->shell_count[p]; ++q) part_shell[k++] = p;
+        for (int q = 0; q < d->shell_count[p]; ++q) {
+            part_col[k] = p * m.nbr_ps + q;
+            part_shell[k++] = p;
+        }
     // balls: the sites of one flip's ball_l on this cell (window, then dilated by the partner offsets l times, as
     // the kernel builds it); the union of two flips holds at most twice as many, and never more than the cell
     {
@@ -1390,6 +1590,24 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
             for (int f = 0; f < 2; ++f)
                 for (int k = 0; k < m.W0; ++k) win_slot[(size_t)f * m.W0 + k] = uslot(f, idx.at(site_of(0, d->window + 4 * k)));
             std::vector<int32_t> list, ends;
+            // the zero row of the BF16 layers' input buffers (xb_zrow): one past the largest ball_0 .. ball_{L-1}
+            int64_t nbr_zrow = 1;
+            for (int l = 0; l < m.L; ++l) nbr_zrow = std::max<int64_t>(nbr_zrow, m.ball_cap[l]);
+            std::vector<uint16_t> nbr;
+            const bool nbr_ok = nbr_zrow < 0xFFFF;
+            for (int l = 0; l < m.L && nbr_ok; ++l) {
+                // per target of ball_l: the source slot of each partner column, the zero row where none
+                const int64_t nsrc = 2 * n1[l], ntgt = 2 * n1[l + 1];
+                m.pre_nbr_off[l] = (int64_t)nbr.size();
+                nbr.resize(nbr.size() + (size_t)(ntgt * m.nbr_w), (uint16_t)nbr_zrow);
+                uint16_t *np = nbr.data() + m.pre_nbr_off[l];
+                for (int64_t q = 0; q < nsrc; ++q)
+                    for (int k = 0; k < n_part; ++k) {
+                        const int64_t y = site_of(cur[src_j[q]], d->partners + 4 * k);
+                        np[(size_t)(uslot(src_f[q], idx.at(y)) * m.nbr_w + part_col[k])] = (uint16_t)q;
+                    }
+            }
+            if (nbr_ok) UP(nbr.data(), nbr.size(), &m.pre_nbr);
             for (int l = 0; l < m.L; ++l) {
                 // hits in the kernel's order (source q, partner k), counted by target, then placed stably
                 const int64_t nsrc = 2 * n1[l], ntgt = 2 * n1[l + 1];
@@ -1449,9 +1667,34 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
         hit_max = std::max(hit_max, (int64_t)m.ball_cap[l] * std::max(n_part, 1));
         dh_max = std::max(dh_max, (int64_t)m.ball_cap[l + 1] * m.dd[l]);
     }
-    m.soff_dh0 = s2; s2 = pad4(s2 + (int64_t)m.ball_cap[0] * m.C0);
-    m.soff_pr = s2; s2 = pad4(s2 + pr_max);
+    // BF16 layers: input widths padded to 32, stacked matrices, and two input-row buffers of bf16 rows in the
+    // projection block (rows 16-byte aligned, the second buffer 128-byte aligned)
+    m.fused_ok = m.L > 0;
+    m.xb_stride = 8;
+    m.wf_bytes = 16;
+    int64_t xb_rows = 1;
+    for (int l = 0; l < m.L; ++l) {
+        const int cin = l == 0 ? m.c0_eff : m.cin[l];
+        m.cpad[l] = (cin + 31) & ~31;
+        m.kf[l] = m.cpad[l] * (1 + 2 * m.P);
+        m.xb_stride = std::max(m.xb_stride, m.cpad[l]);
+        const int nout = m.dd[l] * (m.has_gate ? 2 : 1);
+        m.wf_bytes = std::max<int64_t>(m.wf_bytes, (int64_t)nout * (m.kf[l] + 8) * 2);
+        xb_rows = std::max<int64_t>(xb_rows, m.ball_cap[l]);
+        if (m.dd[l] % 16 != 0 || nout > 16 * NN_FMAXT || m.cpad[l] > 64) m.fused_ok = 0;
+    }
+    if (xb_rows >= 0xFFFF) m.fused_ok = 0;       // slots and the zero row must fit the neighbour tables' uint16
+    m.xb_zrow = (int32_t)xb_rows;                // one row past the largest ball, zeroed by every launch
+    m.xb_off1 = ((xb_rows + 1) * m.xb_stride + 63) & ~(int64_t)63;
+    if (m.fused_ok && m.pre_ok && !m.pre_nbr) m.fused_ok = 0;
+    // two bf16 buffers of xb_off1 elements each take xb_off1 float elements
+    const int64_t pr_region = m.fused_ok ? std::max<int64_t>(pr_max, m.xb_off1 + 64) : pr_max;
+    auto pad32 = [](int64_t v) { return (v + 31) & ~(int64_t)31; };
+    m.soff_dh0 = s2; s2 = pad32(s2 + (int64_t)m.ball_cap[0] * m.C0);
+    m.soff_pr = s2; s2 = pad4(s2 + pr_region);
     m.soff_tgt = s2; s2 = pad4(s2 + hit_max);
+    // BF16 layers: the neighbour table of a hash step (ball_cap[L] rows of nbr_w uint16)
+    m.soff_nbr = s2; if (m.fused_ok) s2 = pad4(s2 + ((int64_t)m.ball_cap[m.L] * m.nbr_w + 1) / 2);
     m.soff_list = s2; s2 = pad4(s2 + hit_max);
     for (int l = 0; l < m.L; ++l) {
         m.soff_dpre[l] = s2; s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
@@ -1459,7 +1702,7 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     }
     m.soff_dh[0] = s2; s2 = pad4(s2 + dh_max);
     m.soff_dh[1] = s2; s2 = pad4(s2 + dh_max);
-    m.scratch_elems = s2;
+    m.scratch_elems = pad32(s2);                 // every replica's scratch starts 128-byte aligned
     std::vector<float> mu_f(m.C0), sd_f(m.C0), lin_f(m.C0), w_f(wts.size()), hw_f(std::max(DL, 1));
     for (int i = 0; i < m.C0; ++i) { mu_f[i] = (float)d->mu[i]; sd_f[i] = (float)d->sd[i]; lin_f[i] = (float)d->lin[i]; }
     for (size_t i = 0; i < wts.size(); ++i) w_f[i] = (float)wts[i];
@@ -1476,12 +1719,21 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     for (int p = 0; p < m.P; ++p) sc[p] = d->shell_count[p];
     UP(sc.data(), sc.size(), &m.shell_count);
     UP(part_shell.data(), part_shell.size(), &m.part_shell);
+    UP(part_col.data(), part_col.size(), &m.part_col);
     if (n_part > 0) UP((const int4 *)d->partners, (size_t)n_part, &m.part);
     UP(c0_ch.data(), c0_ch.size(), &m.c0_ch);
     UP(d->mu, (size_t)m.C0, &m.mu); UP(d->sd, (size_t)m.C0, &m.sd); UP(d->lin, (size_t)m.C0, &m.lin);
     UP(wts.data(), wts.size(), &m.wts); UP(hw.data(), hw.size(), &m.head_w);
     UP(mu_f.data(), mu_f.size(), &m.mu_f); UP(sd_f.data(), sd_f.size(), &m.sd_f); UP(lin_f.data(), lin_f.size(), &m.lin_f);
     UP(w_f.data(), w_f.size(), &m.wts_f); UP(hw_f.data(), hw_f.size(), &m.head_w_f);
+    if (m.fused_ok) {
+        std::vector<uint16_t> img;
+        for (int l = 0; l < m.L; ++l) {
+            m.wf_off[l] = (int64_t)img.size();
+            nn_stack_bf16(m, l, w_f.data() + m.off_wcat[l], c0_ch.data(), img);
+        }
+        UP(img.data(), img.size(), &m.wf_img);
+    }
     if (pr_max >= ((int64_t)1 << 31)) return set_err(NCE_ERR_UNSUPPORTED, "projection block too large");
     m.pr_elems = pr_max;
     m.hit_elems = hit_max;
@@ -1490,11 +1742,12 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     for (int l = 0; l < m.L; ++l) {
         const int64_t cin = l == 0 ? m.c0_eff : m.cin[l];
         m.w_elems = std::max<int64_t>(m.w_elems, cin * m.mcat[l]);
-        m.w_elems = std::max<int64_t>(m.w_elems, (((cin + 31) & ~31) * m.mcat[l] + 1) / 2);   // bf16 copy
         if (m.mcat[l] % 16 != 0) m.mfma_ok = 0;
     }
     m.lds_mode32 = nn_lds_mode<float>(m);
     m.lds_mode64 = nn_lds_mode<double>(m);
+    m.lds_mode16 = nn_lds_mode<float>(m, true);
+    if (m.fused_ok && nn_lds_for<float>(m, 0, true).total > 160 * 1024) m.fused_ok = 0;
     // NCE_NN_LDS_MODE_MAX=k (0..3) at model creation keeps at most the first k per-step arrays in LDS (the others in
     // the scratch): every placement computes the same step
     if (const char *cap_env = std::getenv("NCE_NN_LDS_MODE_MAX")) {
@@ -1502,6 +1755,7 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
         if (cap < 0 || cap > 3) return set_err(NCE_ERR_ARG, "NCE_NN_LDS_MODE_MAX must be 0..3");
         m.lds_mode32 = std::min<int32_t>(m.lds_mode32, cap);
         m.lds_mode64 = std::min<int32_t>(m.lds_mode64, cap);
+        m.lds_mode16 = std::min<int32_t>(m.lds_mode16, cap);
     }
     if (nn_lds_for<float>(m, 0).total > 160 * 1024 || nn_lds_for<double>(m, 0).total > 160 * 1024)
         return set_err(NCE_ERR_UNSUPPORTED, "receptive field too large for the LDS hash table");
@@ -1622,14 +1876,18 @@ static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t
         if ((jb.inj_ia != nullptr) != (jb.inj_ib != nullptr) || (jb.inj_ia != nullptr) != (jb.inj_u != nullptr))
             return set_err(NCE_ERR_ARG, "injected streams must be all set or all NULL");
         for (int r = 0; r < jb.n_replicas; ++r) items.push_back(make_int2(j, r));
-        const int mj = precision == NCE_PREC_FP64 ? m.lds_mode64 : m.lds_mode32;
-        mfma = mfma && m.mfma_ok;
+        mfma = mfma && m.fused_ok;
+    }
+    // one LDS placement for the launch: the smallest of its models'
+    for (int j = 0; j < n_jobs; ++j) {
+        const NNModelDev &m = c->nn_models[jobs[j].model];
+        const int mj = precision == NCE_PREC_FP64 ? m.lds_mode64 : mfma ? m.lds_mode16 : m.lds_mode32;
         mode = j == 0 ? mj : std::min(mode, mj);
     }
     for (int j = 0; j < n_jobs; ++j) {
         const NNModelDev &m = c->nn_models[jobs[j].model];
         lds = std::max(lds, precision == NCE_PREC_FP64 ? nn_lds_for<double>(m, mode).total
-                                                       : nn_lds_for<float>(m, mode).total);
+                                                       : nn_lds_for<float>(m, mode, mfma).total);
     }
     if (lds > 160 * 1024) return set_err(NCE_ERR_UNSUPPORTED, "receptive field does not fit in LDS");
     const size_t jb_bytes = align16(sizeof(nce_nn_job) * n_jobs);
@@ -1660,9 +1918,7 @@ static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t
     if (mfma) {
         switch (mode) {
             case 0: return launch(nn_chunk_kernel<float, 0, true>);
-            case 1: return launch(nn_chunk_kernel<float, 1, true>);
-            case 2: return launch(nn_chunk_kernel<float, 2, true>);
-            default: return launch(nn_chunk_kernel<float, 3, true>);
+            default: return launch(nn_chunk_kernel<float, 1, true>);
         }
     }
     switch (mode) {
