@@ -165,6 +165,8 @@ struct NNModelDev {
     const int32_t *part_col;     // [n_part] column of each partner in a neighbour-table row
     const uint16_t *pre_nbr;
     int64_t pre_nbr_off[NN_MAXL], soff_nbr;
+    int64_t nbr_soff[NN_MAXL];   // hash steps: layer l's table at soff_nbr + nbr_soff[l] (uint16 elements)
+    int64_t soff_pub;            // split replicas: slots published by the part that builds a hash step's balls
     int64_t xb_off1;             // the BF16 layers' two input-row buffers (bf16 [rows][xb_stride]) start at the
                                  // projection block; the second this many bf16 elements after the first
     int64_t off_pre[NN_MAXL], off_gpre[NN_MAXL], cache_elems;
@@ -215,6 +217,8 @@ struct nce_ctx {
     std::vector<Partial> partials;
     // layer-0 rows of NeuralCE full recomputes for models whose caches do not keep them, one buffer per stream
     std::vector<Partial> h0_rows;
+    // barrier counters and energy partials of NeuralCE replicas split over several workgroups, one buffer per stream
+    std::vector<Partial> split_mem;
     std::vector<NNModelDev> nn_models;
     void *d_nn_models = nullptr;
     int cap_nn_models = 0;
@@ -1496,6 +1500,8 @@ extern "C" void nce_ctx_destroy(nce_ctx *c) {
     for (nce_ctx::Partial &pb : c->partials)
         if (pb.p) (void)hipFree(pb.p);
     for (nce_ctx::Partial &pb : c->h0_rows)
+        if (pb.p) (void)hipFree(pb.p);
+    for (nce_ctx::Partial &pb : c->split_mem)
         if (pb.p) (void)hipFree(pb.p);
     if (c->d_nn_models) (void)hipFree(c->d_nn_models);
     for (nce_ctx::StageRing *ring : c->rings) {

@@ -423,13 +423,32 @@ constexpr int NN_FCT = 4;                // BF16 layers: output tiles of 16 colu
 constexpr int NN_FCH = 4;                // BF16 layers: groups of four neighbour-table columns loaded together
 constexpr int NN_FMAXT = 8;              // BF16 layers: at most 8 output tiles (D (1 + gate) <= 128)
 
-template <typename Real, int MODE, bool MFMA>
+// A replica over S workgroups (BF16 layers, cooperative launch of n_items * S workgroups, n_items a multiple of 8):
+// workgroup b runs part (b / 8) % S of item (b / 8 / S) * 8 + b % 8, so the S parts of an item share one XCD (the
+// dispatcher places workgroup b on XCD b % 8; each workgroup checks its XCC id) and its L2.  They exchange rows
+// through that L2: a group barrier drains the stores, counts on a per-item counter (relaxed atomics at agent scope,
+// performed in the L2) and, after its bounded wait, invalidates the CU's L1.  A wait that runs out, or a workgroup
+// on another XCD, sets the abort flag: no later barrier waits, and the replica's energy is returned as NaN.
+struct NNSplit {
+    int32_t S;                   // parts per item (1: one workgroup per replica)
+    uint32_t *bar;               // [n_items] barrier counters, zero at launch
+    double *part;                // [n_items][S] energy partials of the current step
+};
+constexpr uint32_t NN_SPLIT_SPIN = 1u << 22;       // polls of a barrier counter before the wait gives up (~0.1 s)
+
+template <typename Real, int MODE, bool MFMA, bool SPLIT = false>
 __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__restrict__ models,
                                                          const nce_nn_job *__restrict__ jobs,
                                                          const int2 *__restrict__ items, int64_t n_steps,
-                                                         double *__restrict__ eval_dE) {
+                                                         double *__restrict__ eval_dE, const NNSplit sp) {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int2 it = items[blockIdx.x];
+    int item = blockIdx.x, prt = 0;
+    if constexpr (SPLIT) {
+        const int local = blockIdx.x >> 3;
+        prt = local % sp.S;
+        item = (local / sp.S) * 8 + (blockIdx.x & 7);
+    }
+    const int2 it = items[item];
     const nce_nn_job job = jobs[it.x];
     const NNModelDev &m = models[job.model];
     const int r = it.y;
@@ -457,7 +476,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     }
     for (int k = tid; k < m.c0_eff; k += NN_BS) s_c0[k] = m.c0_ch[k];
     for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
-    if (tid == 0) misc[0] = 0;
+    if (tid == 0) {
+        misc[0] = 0;
+        misc[1] = 0;                                   // abort flag of a split replica
+        if constexpr (SPLIT)
+            if (__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) != (blockIdx.x & 7u)) misc[1] = 1;   // XCC_ID
+    }
     if constexpr (MFMA) {
         // the zero row of both input-row buffers of the BF16 layers (read for absent hits and self terms)
         NCE_GLOBAL __bf16 *xb = (NCE_GLOBAL __bf16 *)(gptr((Real *)job.scratch) + (int64_t)r * m.scratch_elems +
@@ -498,6 +522,35 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int *counts = misc + 4;                            // slots of ball_0 .. ball_L this step (same in every thread)
     bool w_staged = false;                             // the projection matrix in LDS is the current layer's
     bool pre_in_list = false;                          // list and cnt hold the fixed table (one-layer models)
+    // group barrier of the S parts of a split replica (a workgroup barrier when S = 1)
+    uint32_t nbar = 0;
+    auto gbar = [&]() {
+        if constexpr (SPLIT) {
+            __builtin_amdgcn_s_waitcnt(0);             // this wave's stores have reached the L2
+            __syncthreads();
+            ++nbar;
+            if (tid == 0 && !misc[1]) {
+                uint32_t *c = sp.bar + item;
+                __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t want = nbar * (uint32_t)sp.S;
+                uint32_t polls = 0;
+                while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++polls > NN_SPLIT_SPIN) {
+                        misc[1] = 1;
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            asm volatile("buffer_inv sc0" ::: "memory");   // the other parts' rows are read from the L2
+        } else {
+            __syncthreads();
+        }
+    };
+    const int S = SPLIT ? sp.S : 1;
+    const bool builder = prt == 0;                     // part that builds a hash step's balls and tables
+    auto n_own = [&](int n) { return (n - prt + S - 1) / S; };   // of rows 0 .. n-1, those of this part: prt + S k
 
     NNProp *s_la = (NNProp *)(smem + lay.la);
     for (int64_t t = 0; t < n_steps; ++t) {
@@ -540,7 +593,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 const int site = (int)nn_site(m, k < m.W0 ? ia : ib, make_int4(0, wk[1], wk[2], wk[3]));
                 if (fixed) {
                     w_pos[k] = gptr(m.pre_win)[k];
-                } else {
+                } else if (builder) {
                     bool claimed;
                     const int pos = nn_hash_claim(keys, mask, site, &claimed);
                     if (claimed) {
@@ -561,10 +614,77 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 }
             }
             __syncthreads();
+            if constexpr (MFMA) {
+                // BF16 layers: every ball and every layer's neighbour table before the layer-0 change
+                if (fixed) {
+                    for (int l = 0; l <= L; ++l) counts[l] = 2 * m.pre_n[l];
+                } else if (builder) {
+                    counts[0] = misc[0];
+                    for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = vals[w_pos[k]];
+                    // hit i = (q, kk) = (i / n_part, i % n_part), stepped without divisions
+                    const int q_0 = n_part > 0 ? tid / n_part : 0, kk_0 = tid - q_0 * n_part;
+                    const int dq = n_part > 0 ? NN_BS / n_part : 0, dk = NN_BS - dq * n_part;
+                    for (int l = 0; l < L; ++l) {
+                        // partners of ball_{l-1}: claim (new sites get the next slots), then the neighbour table
+                        // of ball_l (every column the zero row, then each hit's source in its partner's column; a
+                        // target and a partner have at most one source)
+                        const int nh = counts[l] * n_part;
+                        int q = q_0, kk = kk_0;
+                        for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
+                            if (kk >= n_part) { kk -= n_part; ++q; }
+                            const int site = (int)nn_site(m, slot_site[q], s_part[kk]);
+                            bool claimed;
+                            const int pos = nn_hash_claim(keys, mask, site, &claimed);
+                            if (claimed) {
+                                const int sl = atomicAdd(&misc[0], 1);
+                                vals[pos] = sl;
+                                slot_site[sl] = site;
+                            }
+                            tgt[i] = pos;
+                        }
+                        __syncthreads();
+                        const int nb = misc[0];
+                        counts[l + 1] = nb;
+                        NCE_GLOBAL uint32_t *nb32 = (NCE_GLOBAL uint32_t *)((NCE_GLOBAL uint16_t *)(scr + m.soff_nbr) +
+                                                                            m.nbr_soff[l]);
+                        const uint32_t z2 = (uint32_t)m.xb_zrow * 0x10001u;
+                        for (int e = tid; e < (nb * m.nbr_w) >> 1; e += NN_BS) nb32[e] = z2;
+                        __syncthreads();
+                        NCE_GLOBAL uint16_t *nbt = (NCE_GLOBAL uint16_t *)(scr + m.soff_nbr) + m.nbr_soff[l];
+                        q = q_0; kk = kk_0;
+                        for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
+                            if (kk >= n_part) { kk -= n_part; ++q; }
+                            nbt[(int64_t)vals[tgt[i]] * m.nbr_w + s_part_col[kk]] = (uint16_t)q;
+                        }
+                        __syncthreads();
+                    }
+                }
+                if (SPLIT && !fixed) {
+                    // the builder publishes the slots (counts, window slots, sites); the other parts copy them
+                    NCE_GLOBAL int *pub = (NCE_GLOBAL int *)(scr + m.soff_pub);
+                    const int npub = L + 1 + 2 * m.W0 + counts[L];
+                    if (builder)
+                        for (int k = tid; k < npub; k += NN_BS)
+                            pub[k] = k <= L ? counts[k] : k < L + 1 + 2 * m.W0 ? w_pos[k - L - 1]
+                                                                               : slot_site[k - L - 1 - 2 * m.W0];
+                    gbar();
+                    if (!builder) {
+                        // (clamped into range, so that rows of an aborted step still address memory in bounds)
+                        for (int k = tid; k <= L; k += NN_BS) counts[k] = max(0, min(pub[k], m.ball_cap[k]));
+                        __syncthreads();
+                        for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = max(0, min(pub[L + 1 + k], counts[0] - 1));
+                        for (int k = tid; k < counts[L]; k += NN_BS)
+                            slot_site[k] = max(0, min(pub[L + 1 + 2 * m.W0 + k], (int)m.n_sites - 1));
+                    }
+                }
+                __syncthreads();
+                for (int k = tid; k < counts[0] * C0; k += NN_BS) dh0[k] = (Real)0;
+            } else {
             counts[0] = fixed ? 2 * m.pre_n[0] : misc[0];
             for (int k = tid; k < counts[0] * C0; k += NN_BS) dh0[k] = (Real)0;
             if (!fixed)
                 for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = vals[w_pos[k]];
+            }
             __syncthreads();
             // layer-0 change of flip 1, then of flip 2 with flip 1 applied
             for (int f = 0; f < 2; ++f) {
@@ -595,6 +715,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         for (int q = 0; q < 4; ++q) {
                             if (xs[q] < 0) continue;
                             const int s = w_pos[xs[q]];
+                            if ((s & (S - 1)) != prt) continue;   // a split replica's part keeps its own rows
                             atomicAdd((Real *)&dh0[(int64_t)s * C0 + col], v);
                         }
                     }
@@ -603,11 +724,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             __syncthreads();
             double part = 0.0;
             {
-                // channel of element k = k % C0, stepped without divisions
+                // the layer-0 term over this part's rows (all rows unless the replica is split); element k of those
+                // rows is channel k % C0, stepped without divisions
                 const int dcs = NN_BS % C0;
                 int c = tid % C0;
-                for (int k = tid; k < counts[0] * C0; k += NN_BS) {
-                    part += (double)(dh0[k] * lin[c]);
+                for (int k = tid; k < n_own(counts[0]) * C0; k += NN_BS) {
+                    const int so = S == 1 ? 0 : k / C0;
+                    part += (double)(dh0[(int64_t)(S == 1 ? k : (prt + so * S) * C0 + c)] * lin[c]);
                     c += dcs;
                     c -= c >= C0 ? C0 : 0;
                 }
@@ -616,8 +739,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 // the first layer's input rows: the used channels of the layer-0 change as bf16, zero up to cpad[0]
                 const int n8 = m.cpad[0] >> 3;
                 NCE_GLOBAL __bf16 *x0 = (NCE_GLOBAL __bf16 *)(scr + m.soff_pr);
-                for (int k = tid; k < counts[0] * n8; k += NN_BS) {
-                    const int s = k / n8, c0 = (k - s * n8) * 8;
+                for (int k = tid; k < n_own(counts[0]) * n8; k += NN_BS) {   // (a split replica: alternate rows)
+                    const int so = k / n8, c0 = (k - so * n8) * 8, s = prt + so * S;
                     nn_bf16x8 v;
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
@@ -626,6 +749,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     }
                     *(NCE_GLOBAL nn_bf16x8 *)(x0 + (int64_t)s * m.xb_stride + c0) = v;
                 }
+                gbar();                                // every part's input rows are in place
             }
             // hidden layers: projection of the previous layer's change, partner hits sorted by target, gather
             for (int l = 0; l < L; ++l) {
@@ -774,13 +898,17 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 }
                 }
                 // (2) partner sites of ball_l, (3) the hits sorted by target: list holds (source slot << 8 | shell)
-                // grouped by target, cnt[s] the end of target s's range
+                // grouped by target, cnt[s] the end of target s's range (the BF16 layers built their balls and
+                // neighbour tables before the layer-0 change)
                 const int nh = prev * n_part;
                 int nb;
-                if (fixed) {
+                if constexpr (MFMA) {
+                    nb = counts[l + 1];
+                    __syncthreads();                     // the stacked matrix is staged
+                } else if (fixed) {
                     nb = 2 * m.pre_n[l + 1];
                     counts[l + 1] = nb;
-                    if (!pre_in_list && !MFMA) {        // (the BF16 layers read the neighbour tables in place)
+                    if (!pre_in_list) {
                         const NCE_GLOBAL int *hl = gptr(m.pre_list) + m.pre_list_off[l];
                         const NCE_GLOBAL int *he = gptr(m.pre_end) + m.pre_end_off[l];
                         for (int i = tid; i < nh; i += NN_BS) list[i] = hl[i];
@@ -810,21 +938,6 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 __syncthreads();
                 nb = misc[0];
                 counts[l + 1] = nb;
-                if constexpr (MFMA) {
-                    // BF16 layers: the neighbour table of ball_l (every column the zero row, then each hit's source
-                    // in its partner's column; a target and a partner have at most one source)
-                    NCE_GLOBAL uint32_t *nb32 = (NCE_GLOBAL uint32_t *)(scr + m.soff_nbr);
-                    const uint32_t z2 = (uint32_t)m.xb_zrow * 0x10001u;
-                    for (int e = tid; e < (nb * m.nbr_w) >> 1; e += NN_BS) nb32[e] = z2;
-                    __syncthreads();
-                    NCE_GLOBAL uint16_t *nbt = (NCE_GLOBAL uint16_t *)(scr + m.soff_nbr);
-                    q = q_0; kk = kk_0;
-                    for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
-                        if (kk >= n_part) { kk -= n_part; ++q; }
-                        nbt[(int64_t)vals[tgt[i]] * m.nbr_w + s_part_col[kk]] = (uint16_t)q;
-                    }
-                    __syncthreads();
-                } else {
                 for (int s = tid; s < nb; s += NN_BS) cnt[s] = 0;
                 __syncthreads();
                 for (int i = tid; i < nh; i += NN_BS) {
@@ -842,7 +955,6 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     list[pos] = (q << 8) | s_part_shell[kk];
                 }
                 __syncthreads();
-                }
                 }
                 // (4) change of the pre-activations, new activations, change of the layer output
                 NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
@@ -872,7 +984,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     (const NCE_GLOBAL char *)((const NCE_GLOBAL __bf16 *)(scr + m.soff_pr) + (l & 1) * m.xb_off1);
                 NCE_GLOBAL __bf16 *xout = (NCE_GLOBAL __bf16 *)(scr + m.soff_pr) + ((l + 1) & 1) * m.xb_off1;
                 const NCE_GLOBAL uint16_t *nbt = fixed ? gptr(m.pre_nbr) + m.pre_nbr_off[l]
-                                                       : (const NCE_GLOBAL uint16_t *)(scr + m.soff_nbr);
+                                                       : (const NCE_GLOBAL uint16_t *)(scr + m.soff_nbr) + m.nbr_soff[l];
                 const int nbw = m.nbr_w, nps = m.nbr_ps;
                 const int lane = tid & 63, g = lane >> 4, tl = lane & 15;
                 const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -884,7 +996,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 auto layer = [&]<int NSB>() {
                     constexpr int NZ = 8 * NSB;
                     const uint32_t cob = 16u * (uint32_t)g;   // byte offset of the lane's channels in step 0
-                    for (int task = wave; task < ntask; task += NN_BS / 64) {
+                    for (int task = prt * (NN_BS / 64) + wave; task < ntask; task += S * (NN_BS / 64)) {
                         const int tile = task / ngrp, c0t = (task - tile * ngrp) * NN_FCT;
                         const int t = tile * 16 + tl;
                         const bool tv = t < nb;
@@ -948,8 +1060,10 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
 #pragma unroll
                                 for (int cc = 0; cc < NN_FCH; ++cc) {
                                     if (c0 + cc >= nch) break;
-                                    const int rw[4] = {(int)(e4[cc].x & 0xffffu), (int)(e4[cc].x >> 16),
-                                                       (int)(e4[cc].y & 0xffffu), (int)(e4[cc].y >> 16)};
+                                    // (a table entry never exceeds the zero row; the clamp keeps the rows of an
+                                    // aborted split step in bounds)
+                                    const int rw[4] = {min((int)(e4[cc].x & 0xffffu), zrow), min((int)(e4[cc].x >> 16), zrow),
+                                                       min((int)(e4[cc].y & 0xffffu), zrow), min((int)(e4[cc].y >> 16), zrow)};
                                     nn_bf16x8 v[4][NSB];
 #pragma unroll
                                     for (int u = 0; u < 4; ++u)
@@ -1282,20 +1396,35 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         else dh[k] = dhv;
                     }
                 }
-                __syncthreads();
+                if constexpr (MFMA) {
+                    if (l + 1 < L) gbar();               // this layer's output rows are in place
+                } else {
+                    __syncthreads();
+                }
             }
             w_staged = L == 1;                           // one layer: its matrix stays in LDS for the launch
-            const double diff = nn_block_sum<double>(part, red);
+            double diff;
+            if constexpr (SPLIT) {
+                // every part's partial sum, then the same sum in every part (fixed order)
+                const double mine = nn_block_sum<double>(part, red);
+                if (tid == 0) sp.part[(int64_t)item * S + prt] = mine;
+                gbar();
+                diff = 0.0;
+                for (int q = 0; q < S; ++q) diff += sp.part[(int64_t)item * S + q];
+            } else {
+                diff = nn_block_sum<double>(part, red);
+            }
             const double dE = (m.readout == 0 ? diff / (double)m.nA : diff);
             const double dE_eV = dE * m.de_scale;
             // evaluation only (nce_nn_delta): report the change, leave every state buffer untouched
             const bool accept = eval_dE == nullptr && ((dE_eV <= 0.0) || (u < exp(-beta * dE_eV)));
             if (eval_dE && tid == 0) eval_dE[blockIdx.x] = dE;
             if (accept) {
+                // (the parts of a split replica update alternate rows)
                 if (m.cache_h0)
-                    for (int k = tid; k < counts[0] * C0; k += NN_BS) {
-                        const int s = k / C0, c = k - s * C0;
-                        cache[(int64_t)slot_site[s] * C0 + c] += dh0[k];
+                    for (int k = tid; k < n_own(counts[0]) * C0; k += NN_BS) {
+                        const int so = k / C0, c = k - so * C0, s = prt + so * S;
+                        cache[(int64_t)slot_site[s] * C0 + c] += dh0[(int64_t)s * C0 + c];
                     }
                 for (int l = 0; l < L; ++l) {
                     const int D = m.dd[l];
@@ -1309,10 +1438,10 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         for (int g = 0; g < (m.has_gate ? 2 : 1); ++g) {
                             NCE_GLOBAL Real *cbase = cache + (g == 0 ? m.off_pre[l] : m.off_gpre[l]);
                             const NCE_GLOBAL Real *dbase = g == 0 ? dpre : dgpre;
-                            for (int k = tid; k < counts[l + 1] * D4; k += NN_BS) {
-                                const int s = d4sh >= 0 ? k >> d4sh : k / D4, d = (k - s * D4) * 4;
+                            for (int k = tid; k < n_own(counts[l + 1]) * D4; k += NN_BS) {
+                                const int so = d4sh >= 0 ? k >> d4sh : k / D4, d = (k - so * D4) * 4, s = prt + so * S;
                                 NCE_GLOBAL Real *cp = cbase + (int64_t)slot_site[s] * D + d;
-                                const V4 dv = nn_ld4<Real>(dbase + 4 * (int64_t)k);
+                                const V4 dv = nn_ld4<Real>(dbase + (int64_t)s * D + d);
                                 V4 c = nn_ld4<Real>(cp);
 #pragma unroll
                                 for (int e = 0; e < 4; ++e) c.v[e] += dv.v[e];
@@ -1321,11 +1450,11 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         }
                     } else {
                         const int dsh = (D & (D - 1)) == 0 ? __builtin_ctz((unsigned)D) : -1;
-                        for (int k = tid; k < counts[l + 1] * D; k += NN_BS) {
-                            const int s = dsh >= 0 ? k >> dsh : k / D, d = k - s * D;
+                        for (int k = tid; k < n_own(counts[l + 1]) * D; k += NN_BS) {
+                            const int so = dsh >= 0 ? k >> dsh : k / D, d = k - so * D, s = prt + so * S;
                             const int64_t o = (int64_t)slot_site[s] * D + d;
-                            cache[m.off_pre[l] + o] += dpre[k];
-                            if (m.has_gate) cache[m.off_gpre[l] + o] += dgpre[k];
+                            cache[m.off_pre[l] + o] += dpre[(int64_t)s * D + d];
+                            if (m.has_gate) cache[m.off_gpre[l] + o] += dgpre[(int64_t)s * D + d];
                         }
                     }
                 }
@@ -1348,15 +1477,24 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 if (tid == 0) misc[0] = 0;
             }
         }
-        if (job.traj && eval_dE == nullptr && ((t + 1) % job.sample_interval) == 0 && tid == 0)
+        if (job.traj && eval_dE == nullptr && ((t + 1) % job.sample_interval) == 0 && tid == 0 && prt == 0)
             job.traj[((t + 1) / job.sample_interval - 1) * job.n_replicas + r] = E;
         __syncthreads();
     }
-    if (tid == 0 && eval_dE == nullptr) {
+    if (tid == 0 && eval_dE == nullptr && prt == 0) {
+        // a split replica whose parts lost each other returns NaN (its steps ran without waiting)
+        if (SPLIT && misc[1]) E = __builtin_nan("");
         job.energy[r] = E;
         job.n_accept[r] += nacc;
         job.step[r] = (int64_t)(step0 + (uint64_t)n_steps);
     }
+}
+
+// barrier counters of a split launch start at zero (a fill kernel: the launch may be captured, and memset nodes in
+// captured graphs are avoided)
+__global__ void nn_zero_u32(uint32_t *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ── host side ──────────────────────────────────────────────────────────────────────────────────────────────
@@ -1693,8 +1831,18 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     m.soff_dh0 = s2; s2 = pad32(s2 + (int64_t)m.ball_cap[0] * m.C0);
     m.soff_pr = s2; s2 = pad4(s2 + pr_region);
     m.soff_tgt = s2; s2 = pad4(s2 + hit_max);
-    // BF16 layers: the neighbour table of a hash step (ball_cap[L] rows of nbr_w uint16)
-    m.soff_nbr = s2; if (m.fused_ok) s2 = pad4(s2 + ((int64_t)m.ball_cap[m.L] * m.nbr_w + 1) / 2);
+    // BF16 layers: the neighbour tables of a hash step (layer l: ball_cap[l + 1] rows of nbr_w uint16) and the slots
+    // a split replica's builder publishes (counts, window slots, sites)
+    m.soff_nbr = s2;
+    if (m.fused_ok) {
+        int64_t nt = 0;
+        for (int l = 0; l < m.L; ++l) {
+            m.nbr_soff[l] = nt;
+            nt += ((int64_t)m.ball_cap[l + 1] * m.nbr_w + 7) & ~(int64_t)7;
+        }
+        s2 = pad4(s2 + (nt + 1) / 2);
+    }
+    m.soff_pub = s2; if (m.fused_ok) s2 = pad4(s2 + m.L + 1 + 2 * (int64_t)m.W0 + m.ball_cap[m.L]);
     m.soff_list = s2; s2 = pad4(s2 + hit_max);
     for (int l = 0; l < m.L; ++l) {
         m.soff_dpre[l] = s2; s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
@@ -1900,13 +2048,49 @@ static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t
     const nce_nn_job *d_jobs = (const nce_nn_job *)dev;
     const int2 *d_items = (const int2 *)((unsigned char *)dev + jb_bytes);
     const NNModelDev *dm = (const NNModelDev *)c->d_nn_models;
+    const NNSplit none{1, nullptr, nullptr};
     auto launch = [&](auto kernel) -> int {
         if (int rc = allow_lds(kernel, lds)) return rc;
         hipLaunchKernelGGL(kernel, dim3((unsigned)items.size()), dim3(NN_BS), lds, s, dm, d_jobs, d_items, n_steps,
-                           eval_dE);
+                           eval_dE, none);
         HIP_TRY(hipGetLastError());
         return NCE_OK;
     };
+    // BF16 layers of a one-job launch whose replica count is a multiple of 8: each replica over S workgroups (the
+    // largest of 8, 4, 2 that fits the CUs, at most NCE_NN_SPLIT from the environment; 1 turns it off), one
+    // cooperative launch so that every part is resident.  Launches of several jobs (the served and batched forms,
+    // whose NeuralCE launches share the GPU with CE ticks on another stream) keep one workgroup per replica.
+    if (mfma && !eval_dE && n_jobs == 1) {
+        int smax = 8;
+        if (const char *e = std::getenv("NCE_NN_SPLIT")) smax = std::max(1, std::atoi(e));
+        const int n_items = (int)items.size();
+        int S = 1;
+        if (n_items % 8 == 0 && c->n_cu % 8 == 0)
+            for (int cand = 8; cand >= 2; cand >>= 1)
+                if (cand <= smax && n_items * cand <= c->n_cu) { S = cand; break; }
+        int coop = 0;
+        if (S > 1) HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device));
+        if (S > 1 && coop) {
+            auto kernel = mode == 0 ? nn_chunk_kernel<float, 0, true, true> : nn_chunk_kernel<float, 1, true, true>;
+            if (int rc = allow_lds(kernel, lds)) return rc;
+            int per_cu = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, NN_BS, lds));
+            if (per_cu >= 1) {
+                // counters (uint32, one per item) then the energy partials (double, S per item)
+                const size_t ctr_d = ((size_t)n_items + 1) / 2;
+                double *mem = nullptr;
+                if (int rc = stream_buffer(c->split_mem, s, ctr_d + (size_t)n_items * S, &mem)) return rc;
+                NNSplit sp{S, (uint32_t *)mem, mem + ctr_d};
+                hipLaunchKernelGGL(nn_zero_u32, dim3((n_items + 255) / 256), dim3(256), 0, s, sp.bar, n_items);
+                HIP_TRY(hipGetLastError());
+                void *args[] = {(void *)&dm, (void *)&d_jobs, (void *)&d_items, (void *)&n_steps, (void *)&eval_dE,
+                                (void *)&sp};
+                HIP_TRY(hipLaunchCooperativeKernel((const void *)kernel, dim3((unsigned)(n_items * S)), dim3(NN_BS),
+                                                   args, (unsigned)lds, s));
+                return NCE_OK;
+            }
+        }
+    }
     if (precision == NCE_PREC_FP64) {
         switch (mode) {
             case 0: return launch(nn_chunk_kernel<double, 0, false>);

@@ -504,3 +504,54 @@ def test_silu_network_16cubed_chain_matches_oracle(torch_dev):
           f"{int(np.sum(np.all(occ16 == occ_ref, axis=1)))}/{R} BF16 chains on the FP64 path")
     assert np.max(np.abs(E16 - E16_ref)) < 0.5
     assert np.all(np.abs(d16 - d_ref) <= 0.03 * np.abs(d_ref) + 1e-3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["silu16", "silu5"])
+def test_split_replicas_equal_one_workgroup_per_replica(torch_dev, monkeypatch, kind):
+    """BF16 layers with each replica over S = 2, 4 and 8 workgroups of one XCD (a cooperative launch; the parts
+    exchange their input rows, neighbour tables and energy partials through the XCD's L2) against one workgroup per
+    replica (NCE_NN_SPLIT=1), 8 replicas, the same injected chain of 12 steps: identical occupations, running totals
+    within 1e-6 relative (the energy partials are summed in another grouping), and no NaN (a lost barrier returns
+    NaN).  silu16: the 16,384-site cell, half of the proposals half a cell apart (fixed tables), half uniform (hash
+    tables, built by part 0 and published); silu5: a 500-site cell, where every step builds its balls in the hash
+    table."""
+    import torch
+    from neuralce2_amd.synthetic import fcc_cluster_space, fcc_cell
+    from neuralce2_amd.nce.mc_model import NNModel, silu_network
+    from neuralce2_amd.nce.mcmc import NNJobState
+    from oracle.nce_oracle import _site_lookup
+    cs = fcc_cluster_space((9.0, 6.0, 5.0))
+    P = silu_network(len(cs), 64, 3, 1, seed=2)
+    n = 16 if kind == "silu16" else 5
+    at = fcc_cell(n, 0.5, 1)
+    nn = NNModel(cs, at, P, np.zeros(len(cs)), np.ones(len(cs)), [0.0, 3.3], act="silu", readout="sum")
+    inv, N = nn.geo["inv_perm"], nn.N
+    R, steps = 8, 12
+    rng = np.random.default_rng(5)
+    sig_o = (at.numbers == 78).astype(np.int8)
+    occ_o = np.stack([rng.permutation(sig_o) for _ in range(R)])
+    ia = rng.integers(0, N, (R, steps))
+    ib = rng.integers(0, N, (R, steps))
+    if n % 2 == 0:                                       # (half of an odd cubic cell is not a lattice vector)
+        far = _site_lookup(at)(at.positions[ia.ravel()] + at.cell.sum(0) / 2).reshape(R, steps)
+        ib = np.where(np.arange(steps)[None, :] % 2 == 0, far, ib)
+    u = rng.random((R, steps))
+    betas = (1.0 / (8.617333262e-5 * np.geomspace(600, 2400, R))).astype(np.float32).astype(np.float64)
+    inj = tuple(torch.tensor(x, device=torch_dev) for x in (inv[ia], inv[ib], u))
+    perm = nn.geo["perm"]
+    runs = {}
+    for S in (1, 2, 4, 8):
+        monkeypatch.setenv("NCE_NN_SPLIT", str(S))
+        st = NNJobState(nn, occ_o[:, perm], betas, precision=2, sample_interval=1, samples_per_chunk=steps)
+        st.run(steps, inj)
+        torch.cuda.synchronize()
+        runs[S] = (st.occ.cpu().numpy(), st.energy.cpu().numpy(), st.traj.cpu().numpy())
+    occ1, E1, T1 = runs[1]
+    assert np.all(np.isfinite(E1))
+    for S in (2, 4, 8):
+        occ, E, T = runs[S]
+        assert np.all(np.isfinite(E)), (S, E)
+        assert np.array_equal(occ, occ1), S
+        assert np.allclose(E, E1, rtol=1e-6, atol=1e-6), (S, np.abs(E - E1).max())
+        assert np.allclose(T, T1, rtol=1e-6, atol=1e-6), S
