@@ -198,6 +198,7 @@ def test_bf16_mfma_energy_change(torch_dev, ptni_nce, kind):
         ref.append(o.energy(s1) - o.energy(occ_o[r]))
     ref = np.array(ref)
     err = np.abs(dE - ref)
+    print(f"BF16 single-step dE ({kind}): max relative error {np.max(err / np.maximum(np.abs(ref), 1e-12)):.2e}")
     assert np.all(err <= 3e-2 * np.abs(ref) + 1e-5), (err, ref)
     # chain: running total against a full recompute of the caches
     steps = 30
