@@ -34,9 +34,13 @@ B_ALG_PER_ATTEMPT = 284           # bytes: two 141-site int8 windows + two writt
 KB = 8.617333e-5
 PUBLISHED_H100 = 4_605_830.0      # reference: CE attempts/s at N = 1,048,576, R = 256 on H100 (pure step loop)
 # HBM bytes per swap attempt moved by the chunk kernel, from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per
-# dispatch (factors from a 1 GiB device copy), for the workload and precision they were measured on:
-# profiles/ce_chunk_spec_kernel_fcc64cubic_1m_r256.md and profiles/nn_chunk_kernel_ptni_fcc16cubic_r32.md
-MEASURED_TRAFFIC_PER_ATTEMPT = {("ce_1m", "mixed"): 3461.0, ("nce_ptni_16k", "mixed"): 82177.0}
+# dispatch (factors from a 1 GiB device copy), for the workload and precision they were measured on, with the
+# profile summary that holds the counts (the counters cannot run inside the timed region)
+MEASURED_TRAFFIC_PER_ATTEMPT = {("ce_1m", "mixed"): 3461.0, ("nce_ptni_16k", "mixed"): 82177.0,
+                                ("nce_16k", "bf16"): 2.583e6}
+TRAFFIC_PROFILE = {("ce_1m", "mixed"): "profiles/ce_chunk_spec_kernel_fcc64cubic_1m_r256.md",
+                   ("nce_ptni_16k", "mixed"): "profiles/nn_chunk_kernel_ptni_fcc16cubic_r32.md",
+                   ("nce_16k", "bf16"): "profiles/nn_chunk_kernel_silu3x64_fcc16cubic_r64_bf16.md"}
 
 CONFIGS = {
     "ce_1m": dict(n=64, x=0.25, R=256, Tmin=300.0, Tmax=1400.0, exchange=5000, precision="mixed"),
@@ -310,6 +314,7 @@ def main():
                              "traffic": (MEASURED_TRAFFIC_PER_ATTEMPT[(args.config, cfg["precision"])] * res["R"]
                                          * args.chunk
                                          if (args.config, cfg["precision"]) in MEASURED_TRAFFIC_PER_ATTEMPT else None),
+                             "traffic_from": TRAFFIC_PROFILE.get((args.config, cfg["precision"])),
                              "unit": "TFLOP/s", "frac": res["achieved"] / res["peak"],
                              "kernel": "nn_chunk_kernel", "kernel_ms_avg": float(res["k_ms"].mean()),
                              "flops_per_attempt": res["flops_attempt"]},
@@ -385,6 +390,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (MEASURED_TRAFFIC_PER_ATTEMPT[(args.config, cfg["precision"])] * R * chunk
                                  if (args.config, cfg["precision"]) in MEASURED_TRAFFIC_PER_ATTEMPT else None),
+                     "traffic_from": TRAFFIC_PROFILE.get((args.config, cfg["precision"])),
                      "kernel": "ce_chunk_spec_kernel", "kernel_ms_avg": float(k_ms.mean()),
                      "bytes_per_attempt_alg": B_ALG_PER_ATTEMPT},
         "cpu_baseline": None,
