@@ -99,6 +99,9 @@ def setup(mdl, atoms, seed_idx=0, dtype="float32", table=None, device=0, cache_m
     seed_idx < 0 selects the seed ensemble (mean energy over every saved parameter set), run as one network whose
     width is the sum of the seeds' widths (model_io.ensemble_params).  cache_mode 'h0' / 'proj' / None (the
     reference's choice by per-site bytes, default_cache_mode), as setup_implicit's (nce/mc_model.py:175,202-206)."""
+    if int(mdl.get("spin", 0) or 0):
+        raise NotImplementedError("spin models (spin channels in layer 0, spin moves; reference nce/spin_features.py) "
+                                  "are not supported by the gfx950 local update")
     cs = ClusterSpace(mdl["primitive"], list(mdl["cutoffs"]), mdl["chemical_symbols"])
     if seed_idx is not None and int(seed_idx) < 0:
         P, mu, sd = ensemble_params(mdl)

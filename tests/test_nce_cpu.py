@@ -244,6 +244,12 @@ def test_unsupported_parameter_sets_are_refused(model):
     L2 = {k: v for k, v in P["layers"][0].items() if k not in ("gate", "gb")}
     with pytest.raises(NotImplementedError, match="gate"):
         check_supported(dict(P, layers=[P["layers"][0], L2]))
+    # a spin model (the shipped PtNi model marked as one) is refused by setup before anything is built
+    from neuralce2_amd.lattice import read_cif
+    from neuralce2_amd.nce.mc_model import setup
+    at = read_cif(os.path.join(GOLDEN, "init_L10_N48.cif"))
+    with pytest.raises(NotImplementedError, match="spin"):
+        setup(dict(mdl, spin=1, n_spin=12), at)
 
 
 def test_default_cache_mode_follows_the_reference_rule(model):
