@@ -147,6 +147,7 @@ struct NNModelDev {
     // per precision: which per-step arrays live in LDS (else in the scratch): 0 none, 1 the layer-0 change of
     // ball 0, 2 and the sorted partner hits, 3 and the projections
     int32_t lds_mode32, lds_mode64, lds_mode16;
+    int32_t fp32_in_lds;         // the whole FP32 step fits in LDS (placement 3 before any cap from the environment)
     int64_t pr_elems, hit_elems, w_elems;   // largest projection block / hit list / used projection matrix
     int32_t mfma_ok;             // every layer's projection width is a multiple of 16 (FP32 MFMA tiles)
     // BF16 layers (gathered shell sums times the stacked matrix on MFMA): every width D a multiple of 16,

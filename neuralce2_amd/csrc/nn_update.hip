@@ -1895,6 +1895,7 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     m.lds_mode32 = nn_lds_mode<float>(m);
     m.lds_mode64 = nn_lds_mode<double>(m);
     m.lds_mode16 = nn_lds_mode<float>(m, true);
+    m.fp32_in_lds = m.lds_mode32 == 3;
     if (m.fused_ok && nn_lds_for<float>(m, 0, true).total > 160 * 1024) m.fused_ok = 0;
     // NCE_NN_LDS_MODE_MAX=k (0..3) at model creation keeps at most the first k per-step arrays in LDS (the others in
     // the scratch): every placement computes the same step
@@ -2011,7 +2012,13 @@ static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t
     std::vector<int2> items;
     size_t lds = 0;
     int mode = 0;
-    bool mfma = precision == NCE_PREC_BF16;
+    // BF16 runs the fused BF16 layers when a model of the launch does not keep its FP32 projections in LDS (the
+    // wide multi-layer networks); a launch whose every model's FP32 step fits in LDS (the shipped PtNi model) runs
+    // the FP32 kernel, which is faster there (16,384 sites, 32 replicas: 2.46e6 attempts/s against 1.36e6 fused
+    // over 8 workgroups).  NCE_NN_FUSED=1 forces the fused layers, 0 turns them off.
+    bool mfma = precision == NCE_PREC_BF16, wide = false;
+    const char *fused_env = std::getenv("NCE_NN_FUSED");
+    const int fused_force = fused_env ? std::atoi(fused_env) : -1;
     for (int j = 0; j < n_jobs; ++j) {
         const nce_nn_job &jb = jobs[j];
         if (jb.model < 0 || jb.model >= (int)c->nn_models.size()) return set_err(NCE_ERR_ARG, "unknown NeuralCE model");
@@ -2024,8 +2031,10 @@ static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t
         if ((jb.inj_ia != nullptr) != (jb.inj_ib != nullptr) || (jb.inj_ia != nullptr) != (jb.inj_u != nullptr))
             return set_err(NCE_ERR_ARG, "injected streams must be all set or all NULL");
         for (int r = 0; r < jb.n_replicas; ++r) items.push_back(make_int2(j, r));
-        mfma = mfma && m.fused_ok;
+        mfma = mfma && m.fused_ok && fused_force != 0;
+        wide = wide || !m.fp32_in_lds;
     }
+    mfma = mfma && (fused_force == 1 || wide);
     // one LDS placement for the launch: the smallest of its models'
     for (int j = 0; j < n_jobs; ++j) {
         const NNModelDev &m = c->nn_models[jobs[j].model];

@@ -153,11 +153,13 @@ def test_silu_three_layer_network(torch_dev):
 
 
 @pytest.mark.parametrize("kind", ["silu", "ptni"])
-def test_bf16_mfma_energy_change(torch_dev, ptni_nce, kind):
-    """BF16 MFMA projections (FP32 accumulation, FP32 caches): the energy change of one accepted swap per replica
+def test_bf16_mfma_energy_change(torch_dev, ptni_nce, kind, monkeypatch):
+    """BF16 layers on MFMA (FP32 accumulation, FP32 caches): the energy change of one accepted swap per replica
     against the FP64 oracle's difference of full energies, within 3% of |dE| + 1e-5 (model units per site for the
     mean readout, totals for the summed one); after a chain of such steps the running total matches a full recompute
-    within the same bound per step."""
+    within the same bound per step.  The PtNi model (gated, three shells, 12 used channels) takes the fused BF16
+    layers by NCE_NN_FUSED=1 (by default its BF16 requests run the FP32 kernel, whose step fits in LDS)."""
+    monkeypatch.setenv("NCE_NN_FUSED", "1")
     import torch
     from neuralce2_amd.nce.mc_model import NNModel, silu_network
     from neuralce2_amd.nce.mcmc import NNJobState
@@ -542,6 +544,7 @@ def test_split_replicas_equal_one_workgroup_per_replica(torch_dev, monkeypatch, 
     inj = tuple(torch.tensor(x, device=torch_dev) for x in (inv[ia], inv[ib], u))
     perm = nn.geo["perm"]
     runs = {}
+    monkeypatch.setenv("NCE_NN_FUSED", "1")
     for S in (1, 2, 4, 8):
         monkeypatch.setenv("NCE_NN_SPLIT", str(S))
         st = NNJobState(nn, occ_o[:, perm], betas, precision=2, sample_interval=1, samples_per_chunk=steps)
