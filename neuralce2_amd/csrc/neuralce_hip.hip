@@ -161,12 +161,12 @@ struct NNModelDev {
     int64_t wf_off[NN_MAXL];
     // BF16 layers: per target a row of nbr_w = P * nbr_ps uint16 source slots, shell p's partners in columns
     // p * nbr_ps + (index within the shell), the zero row xb_zrow where the partner is outside ball_{l-1}.
-    // Fixed balls: one table per layer in device memory (pre_nbr); hash steps build theirs in the scratch.
+    // Fixed balls: one table per layer in device memory (pre_nbr); hash steps build one for every layer in the
+    // scratch (a layer takes the sources below its ball's size).
     int32_t nbr_w, nbr_ps;
     const int32_t *part_col;     // [n_part] column of each partner in a neighbour-table row
     const uint16_t *pre_nbr;
     int64_t pre_nbr_off[NN_MAXL], soff_nbr;
-    int64_t nbr_soff[NN_MAXL];   // hash steps: layer l's table at soff_nbr + nbr_soff[l] (uint16 elements)
     int64_t soff_pub;            // split replicas: slots published by the part that builds a hash step's balls
     int64_t xb_off1;             // the BF16 layers' two input-row buffers (bf16 [rows][xb_stride]) start at the
                                  // projection block; the second this many bf16 elements after the first

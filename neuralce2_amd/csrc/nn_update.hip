@@ -625,12 +625,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const int q_0 = n_part > 0 ? tid / n_part : 0, kk_0 = tid - q_0 * n_part;
                     const int dq = n_part > 0 ? NN_BS / n_part : 0, dk = NN_BS - dq * n_part;
                     for (int l = 0; l < L; ++l) {
-                        // partners of ball_{l-1}: claim (new sites get the next slots), then the neighbour table
-                        // of ball_l (every column the zero row, then each hit's source in its partner's column; a
-                        // target and a partner have at most one source)
-                        const int nh = counts[l] * n_part;
-                        int q = q_0, kk = kk_0;
-                        for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
+                        // ball_{l+1}: the partners of ball_l's sites, claimed (new sites get the next slots) for
+                        // the sites new in ball_l only (an older site's partners are in ball_l already); the hit
+                        // (q, kk) keeps its table position in tgt[q n_part + kk]
+                        const int q_lo = l == 0 ? 0 : counts[l - 1], q_hi = counts[l];
+                        int q = q_lo + q_0, kk = kk_0;
+                        for (int i = q_lo * n_part + tid; i < q_hi * n_part; i += NN_BS, q += dq, kk += dk) {
                             if (kk >= n_part) { kk -= n_part; ++q; }
                             const int site = (int)nn_site(m, slot_site[q], s_part[kk]);
                             bool claimed;
@@ -643,21 +643,22 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             tgt[i] = pos;
                         }
                         __syncthreads();
-                        const int nb = misc[0];
-                        counts[l + 1] = nb;
-                        NCE_GLOBAL uint32_t *nb32 = (NCE_GLOBAL uint32_t *)((NCE_GLOBAL uint16_t *)(scr + m.soff_nbr) +
-                                                                            m.nbr_soff[l]);
-                        const uint32_t z2 = (uint32_t)m.xb_zrow * 0x10001u;
-                        for (int e = tid; e < (nb * m.nbr_w) >> 1; e += NN_BS) nb32[e] = z2;
-                        __syncthreads();
-                        NCE_GLOBAL uint16_t *nbt = (NCE_GLOBAL uint16_t *)(scr + m.soff_nbr) + m.nbr_soff[l];
-                        q = q_0; kk = kk_0;
-                        for (int i = tid; i < nh; i += NN_BS, q += dq, kk += dk) {
-                            if (kk >= n_part) { kk -= n_part; ++q; }
-                            nbt[(int64_t)vals[tgt[i]] * m.nbr_w + s_part_col[kk]] = (uint16_t)q;
-                        }
-                        __syncthreads();
+                        counts[l + 1] = misc[0];
                     }
+                    // one neighbour table for every layer: row t of ball_L, the source slot of each partner's hit
+                    // (a target and a partner have at most one source), the zero row where none; layer l reads the
+                    // sources below counts[l] (the slots are nested) and takes any other as the zero row
+                    NCE_GLOBAL uint32_t *nb32 = (NCE_GLOBAL uint32_t *)(scr + m.soff_nbr);
+                    const uint32_t z2 = (uint32_t)m.xb_zrow * 0x10001u;
+                    for (int e = tid; e < (counts[L] * m.nbr_w) >> 1; e += NN_BS) nb32[e] = z2;
+                    __syncthreads();
+                    NCE_GLOBAL uint16_t *nbt = (NCE_GLOBAL uint16_t *)(scr + m.soff_nbr);
+                    int q = q_0, kk = kk_0;
+                    for (int i = tid; i < counts[L - 1] * n_part; i += NN_BS, q += dq, kk += dk) {
+                        if (kk >= n_part) { kk -= n_part; ++q; }
+                        nbt[(int64_t)vals[tgt[i]] * m.nbr_w + s_part_col[kk]] = (uint16_t)q;
+                    }
+                    __syncthreads();
                 }
                 if (SPLIT && !fixed) {
                     // the builder publishes the slots (counts, window slots, sites); the other parts copy them
@@ -984,7 +985,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     (const NCE_GLOBAL char *)((const NCE_GLOBAL __bf16 *)(scr + m.soff_pr) + (l & 1) * m.xb_off1);
                 NCE_GLOBAL __bf16 *xout = (NCE_GLOBAL __bf16 *)(scr + m.soff_pr) + ((l + 1) & 1) * m.xb_off1;
                 const NCE_GLOBAL uint16_t *nbt = fixed ? gptr(m.pre_nbr) + m.pre_nbr_off[l]
-                                                       : (const NCE_GLOBAL uint16_t *)(scr + m.soff_nbr) + m.nbr_soff[l];
+                                                       : (const NCE_GLOBAL uint16_t *)(scr + m.soff_nbr);
                 const int nbw = m.nbr_w, nps = m.nbr_ps;
                 const int lane = tid & 63, g = lane >> 4, tl = lane & 15;
                 const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1060,10 +1061,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
 #pragma unroll
                                 for (int cc = 0; cc < NN_FCH; ++cc) {
                                     if (c0 + cc >= nch) break;
-                                    // (a table entry never exceeds the zero row; the clamp keeps the rows of an
-                                    // aborted split step in bounds)
-                                    const int rw[4] = {min((int)(e4[cc].x & 0xffffu), zrow), min((int)(e4[cc].x >> 16), zrow),
-                                                       min((int)(e4[cc].y & 0xffffu), zrow), min((int)(e4[cc].y >> 16), zrow)};
+                                    // (a hash step's table serves every layer: a source at or past prev, outside
+                                    // ball_{l-1}, reads the zero row; the test also keeps the rows of an aborted
+                                    // split step in bounds)
+                                    auto rsel = [&](uint32_t v) -> int { return (int)v < prev ? (int)v : zrow; };
+                                    const int rw[4] = {rsel(e4[cc].x & 0xffffu), rsel(e4[cc].x >> 16),
+                                                       rsel(e4[cc].y & 0xffffu), rsel(e4[cc].y >> 16)};
                                     nn_bf16x8 v[4][NSB];
 #pragma unroll
                                     for (int u = 0; u < 4; ++u)
@@ -1831,17 +1834,10 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     m.soff_dh0 = s2; s2 = pad32(s2 + (int64_t)m.ball_cap[0] * m.C0);
     m.soff_pr = s2; s2 = pad4(s2 + pr_region);
     m.soff_tgt = s2; s2 = pad4(s2 + hit_max);
-    // BF16 layers: the neighbour tables of a hash step (layer l: ball_cap[l + 1] rows of nbr_w uint16) and the slots
+    // BF16 layers: the neighbour table of a hash step (ball_cap[L] rows of nbr_w uint16, every layer's) and the slots
     // a split replica's builder publishes (counts, window slots, sites)
     m.soff_nbr = s2;
-    if (m.fused_ok) {
-        int64_t nt = 0;
-        for (int l = 0; l < m.L; ++l) {
-            m.nbr_soff[l] = nt;
-            nt += ((int64_t)m.ball_cap[l + 1] * m.nbr_w + 7) & ~(int64_t)7;
-        }
-        s2 = pad4(s2 + (nt + 1) / 2);
-    }
+    if (m.fused_ok) s2 = pad4(s2 + ((int64_t)m.ball_cap[m.L] * m.nbr_w + 1) / 2);
     m.soff_pub = s2; if (m.fused_ok) s2 = pad4(s2 + m.L + 1 + 2 * (int64_t)m.W0 + m.ball_cap[m.L]);
     m.soff_list = s2; s2 = pad4(s2 + hit_max);
     for (int l = 0; l < m.L; ++l) {
