@@ -231,6 +231,10 @@ def _run_replicas(mc, betas, n_steps, seed, chunk, log_every, sample_interval, o
     t0 = time.time()
     for c in range(c_first, n_chunks):
         st.run(chunk)
+        # (a BF16 replica run over several workgroups returns NaN if its workgroups lost each other; the check
+        # comes before a full recompute could replace the energy)
+        if not bool(st.energy.isfinite().all().item()):
+            raise RuntimeError(f"[{tag}] chunk {c}: non-finite replica energies after the local-update kernel")
         last = c == n_chunks - 1
         if (c + 1) % anchor_every == 0 or last:
             E_run, E_full = st.anchor()
