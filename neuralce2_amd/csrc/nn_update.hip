@@ -997,7 +997,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 auto layer = [&]<int NSB>() {
                     constexpr int NZ = 8 * NSB;
                     const uint32_t cob = 16u * (uint32_t)g;   // byte offset of the lane's channels in step 0
-                    for (int task = prt * (NN_BS / 64) + wave; task < ntask; task += S * (NN_BS / 64)) {
+                    // (the parts of a split replica interleave, so a layer's last, partial round of tiles is spread over
+                // all of them)
+                for (int task = wave * S + prt; task < ntask; task += S * (NN_BS / 64)) {
                         const int tile = task / ngrp, c0t = (task - tile * ngrp) * NN_FCT;
                         const int t = tile * 16 + tl;
                         const bool tv = t < nb;
@@ -2090,9 +2092,10 @@ static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t
                 HIP_TRY(hipGetLastError());
                 void *args[] = {(void *)&dm, (void *)&d_jobs, (void *)&d_items, (void *)&n_steps, (void *)&eval_dE,
                                 (void *)&sp};
-                HIP_TRY(hipLaunchCooperativeKernel((const void *)kernel, dim3((unsigned)(n_items * S)), dim3(NN_BS),
-                                                   args, (unsigned)lds, s));
-                return NCE_OK;
+                const hipError_t e = hipLaunchCooperativeKernel((const void *)kernel, dim3((unsigned)(n_items * S)),
+                                                                dim3(NN_BS), args, (unsigned)lds, s);
+                if (e == hipSuccess) return NCE_OK;
+                (void)hipGetLastError();               // refused (not co-resident here): one workgroup per replica
             }
         }
     }
