@@ -154,8 +154,9 @@ class _Record:
 
 def _save_checkpoint(path, c_done, st, rec, want, betas):
     """reference keys (mcmc.py:389-399): chunk, sig, spin, key, drift, n_ex_try, n_ex_acc, E_chunk, acc_chunk,
-    drift_log, traj, snaps, snap_chunks, cfg_betas and cfg_*; `key` holds this build's stream state (the replicas'
-    step counters and the exchange count) and n_accept the accepted-move counters."""
+    drift_log, traj, snaps, snap_chunks, spin_snaps, cfg_betas and cfg_*; `key` holds this build's stream state (the
+    replicas' step counters and the exchange count) and n_accept the accepted-move counters.  spin and spin_snaps
+    are empty: spin models are refused at setup."""
     R = st.R
     tmp = path + ".tmp.npz"
     np.savez(tmp, chunk=np.int64(c_done), sig=st.occ.cpu().numpy(), spin=np.zeros(0),
@@ -164,7 +165,7 @@ def _save_checkpoint(path, c_done, st, rec, want, betas):
              n_ex_acc=np.int64(rec.n_ex_acc), E_chunk=np.array(rec.E_chunk).reshape(-1, R),
              acc_chunk=np.array(rec.acc_chunk).reshape(-1, R), drift_log=np.array(rec.drift_log, np.float64),
              traj=rec.traj_array(R), snaps=np.zeros((0, R, st.nn.N), np.int8), snap_chunks=np.zeros(0, np.int64),
-             cfg_betas=np.asarray(betas, np.float64), **{"cfg_" + k: v for k, v in want.items()})
+             spin_snaps=np.zeros((0, R, st.nn.N), np.float16), cfg_betas=np.asarray(betas, np.float64), **{"cfg_" + k: v for k, v in want.items()})
     os.replace(tmp, path)
 
 

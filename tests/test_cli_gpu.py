@@ -11,6 +11,27 @@ from tests.conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
+# the keys of the reference's output files, which a run here must also write (extra keys of this build allowed)
+CE_ENERGIES_KEYS = {"energies", "temperatures", "ladder_id", "n_atoms", "swap_accept_rate",   # pt_mcmc_ce.py:972-975
+                    "exchange_accept_rate", "flip_accept_rate", "E_min", "wall_time"}
+CE_RESULT_KEYS = {"energies", "temperatures", "ladder_id", "ladder_pos", "n_ladders",         # pt_mcmc_ce.py:1003-1018
+                  "sigma_final", "sigma_min", "E_min", "spins_final", "swap_accept_rate", "flip_accept_rate",
+                  "exchange_accept_rate", "wall_time"}
+CE_CHECKPOINT_KEYS = {"chunk", "si", "energy_samples", "key", "sigma", "rep_of_slot", "beta_r",  # pt_mcmc_ce.py:925-930
+                      "cfg_n_steps", "cfg_chunk", "cfg_n_total", "cfg_n_atoms", "cfg_sample_interval", "cfg_seed",
+                      "cfg_temps", "cfg_run_id", "spins", "E", "sigma_min", "E_min", "swap_cum", "flip_cum",
+                      "ex_cum", "step"}
+NCE_ENERGIES_KEYS = {"E_chunk", "acc_chunk", "drift", "drift_eV", "traj", "betas", "temps"}   # nce/mcmc.py:402-404
+NCE_CHECKPOINT_KEYS = {"chunk", "sig", "spin", "key", "drift", "n_ex_try", "n_ex_acc",          # nce/mcmc.py:392-397
+                       "E_chunk", "acc_chunk", "drift_log", "traj", "snaps", "snap_chunks", "spin_snaps",
+                       "cfg_betas", "cfg_n_steps", "cfg_chunk", "cfg_R", "cfg_n_atoms", "cfg_sample_interval",
+                       "cfg_seed", "cfg_anchor_every", "cfg_exchange_every", "cfg_model"}
+
+
+def _keys(path):
+    with np.load(path, allow_pickle=False) as z:
+        return set(z.files)
+
 
 @pytest.fixture(scope="module")
 def torch_dev():
@@ -31,7 +52,8 @@ def test_cli_icet_pt(torch_dev, tmp_path):
     assert 0.0 < s["swap_accept_rate"] <= 1.0
     z = np.load(out / "mcmc_result.npz", allow_pickle=False)
     assert z["energies"].shape == (40, 8) and np.all(np.isfinite(z["energies"]))
-    assert os.path.exists(out / "energies.npz")
+    assert CE_RESULT_KEYS <= set(z.files), CE_RESULT_KEYS - set(z.files)
+    assert CE_ENERGIES_KEYS <= _keys(out / "energies.npz"), CE_ENERGIES_KEYS - _keys(out / "energies.npz")
 
 
 def test_cli_nce_pt_with_init_npz(torch_dev, tmp_path):
@@ -51,6 +73,8 @@ def test_cli_nce_pt_with_init_npz(torch_dev, tmp_path):
     assert s["n_atoms"] == 48 and 0.0 < s["acc_mean"] <= 1.0 and s["drift_max"] < 1e-3
     z = np.load(out / "energies.npz", allow_pickle=False)
     assert z["traj"].shape[0] == 4 and np.all(z["traj"] > -6.1) and np.all(z["traj"] < -5.8)
+    assert NCE_ENERGIES_KEYS <= set(z.files), NCE_ENERGIES_KEYS - set(z.files)
+    assert os.path.exists(out / "final_sigma.npy")                         # nce/mcmc.py:407
 
 
 @pytest.mark.parametrize("dtype", ["float64", "bfloat16"])
@@ -118,6 +142,8 @@ def test_ce_checkpoint_resume_equals_the_uninterrupted_run(torch_dev, tmp_path, 
     with pytest.raises(_Preempted):
         pm.run_pt_mcmc(g, cfg(tmp_path / "ck"))
     assert os.path.exists(tmp_path / "ck" / "checkpoint.npz")
+    have = _keys(tmp_path / "ck" / "checkpoint.npz")
+    assert CE_CHECKPOINT_KEYS <= have, CE_CHECKPOINT_KEYS - have
     with pytest.raises(RuntimeError, match="another run"):
         pm.run_pt_mcmc(g, cfg(tmp_path / "ck", seed=10))
     monkeypatch.undo()
@@ -142,6 +168,8 @@ def test_nce_checkpoint_resume_equals_the_uninterrupted_run(torch_dev, tmp_path,
     with pytest.raises(_Preempted):
         mcmc.run_pt_mc(*args, out_dir=str(tmp_path / "ck"), **kw)
     assert os.path.exists(tmp_path / "ck" / "checkpoint.npz")
+    have = _keys(tmp_path / "ck" / "checkpoint.npz")
+    assert NCE_CHECKPOINT_KEYS <= have, NCE_CHECKPOINT_KEYS - have
     with pytest.raises(RuntimeError, match="another run"):
         mcmc.run_pt_mc(*args, out_dir=str(tmp_path / "ck"), **{**kw, "seed": 5})
     monkeypatch.undo()
