@@ -171,7 +171,7 @@ struct NNModelDev {
     int64_t xb_off1;             // the BF16 layers' two input-row buffers (bf16 [rows][xb_stride]) start at the
                                  // projection block; the second this many bf16 elements after the first
     int64_t off_pre[NN_MAXL], off_gpre[NN_MAXL], cache_elems;
-    int64_t soff_dh0, soff_pr, soff_tgt, soff_list, soff_dpre[NN_MAXL], soff_dgpre[NN_MAXL], soff_dh[2], scratch_elems;
+    int64_t soff_dh0, soff_pr, soff_tgt, soff_list, soff_npre[NN_MAXL], soff_ngpre[NN_MAXL], soff_dh[2], scratch_elems;
 };
 
 struct NNModelHost {

@@ -260,6 +260,7 @@ __global__ void nn_energy_finish_kernel(const NNModelDev *__restrict__ models, i
 constexpr int NN_BS = 1024;                // threads per replica: 16 waves, 4 per SIMD
 constexpr int NN_EMPTY = -1;
 constexpr int NN_RB = 4;                 // rows per thread in the projection
+constexpr int NN_CU = 4;                 // cache rows per thread in flight together at a commit
 constexpr int NN_LA = 8;                 // proposals drawn, and their species read, ahead of the steps (power of 2)
 // one drawn proposal and the species of its two sites at the step it runs
 struct NNProp {
@@ -626,8 +627,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const int dq = n_part > 0 ? NN_BS / n_part : 0, dk = NN_BS - dq * n_part;
                     for (int l = 0; l < L; ++l) {
                         // ball_{l+1}: the partners of ball_l's sites, claimed (new sites get the next slots) for
-                        // the sites new in ball_l only (an older site's partners are in ball_l already); the hit
-                        // (q, kk) keeps its table position in tgt[q n_part + kk]
+                        // the sites new in ball_l only (an older site's partners are in ball_l already)
                         const int q_lo = l == 0 ? 0 : counts[l - 1], q_hi = counts[l];
                         int q = q_lo + q_0, kk = kk_0;
                         for (int i = q_lo * n_part + tid; i < q_hi * n_part; i += NN_BS, q += dq, kk += dk) {
@@ -640,7 +640,6 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                                 vals[pos] = sl;
                                 slot_site[sl] = site;
                             }
-                            tgt[i] = pos;
                         }
                         __syncthreads();
                         counts[l + 1] = misc[0];
@@ -653,10 +652,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     for (int e = tid; e < (counts[L] * m.nbr_w) >> 1; e += NN_BS) nb32[e] = z2;
                     __syncthreads();
                     NCE_GLOBAL uint16_t *nbt = (NCE_GLOBAL uint16_t *)(scr + m.soff_nbr);
+                    // (each hit's partner found again in the hash, every slot now final: LDS reads only)
                     int q = q_0, kk = kk_0;
                     for (int i = tid; i < counts[L - 1] * n_part; i += NN_BS, q += dq, kk += dk) {
                         if (kk >= n_part) { kk -= n_part; ++q; }
-                        nbt[(int64_t)vals[tgt[i]] * m.nbr_w + s_part_col[kk]] = (uint16_t)q;
+                        const int t = nn_hash_find(keys, vals, mask, (int)nn_site(m, slot_site[q], s_part[kk]));
+                        if (t >= 0) nbt[(int64_t)t * m.nbr_w + s_part_col[kk]] = (uint16_t)q;
                     }
                     __syncthreads();
                 }
@@ -958,8 +959,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 __syncthreads();
                 }
                 // (4) change of the pre-activations, new activations, change of the layer output
-                NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
-                NCE_GLOBAL Real *dgpre = scr + m.soff_dgpre[l];
+                // the targets' new pre-activations (cached value + change), copied into the cache at a commit
+                NCE_GLOBAL Real *npre = scr + m.soff_npre[l];
+                NCE_GLOBAL Real *ngpre = scr + m.soff_ngpre[l];
                 NCE_GLOBAL Real *dh = scr + m.soff_dh[l & 1];
                 const NCE_GLOBAL Real *cpre = cache + m.off_pre[l];
                 const NCE_GLOBAL Real *cgpre = cache + m.off_gpre[l];
@@ -1102,16 +1104,19 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             auto out = [&]<int ACT>(int jc, const nn_floatx4 &ap, const nn_floatx4 *ag) {
                                 const V4 pv = nn_ld4<Real>(cpre + so + jc);
                                 const V4 dp = V4{{ap[0], ap[1], ap[2], ap[3]}};
-                                V4 gv = z4, dg = z4, hv;
+                                V4 gv = z4, dg = z4, hv, nv, ng;
                                 if (ag) {
                                     gv = nn_ld4<Real>(cgpre + so + jc);
                                     dg = V4{{(*ag)[0], (*ag)[1], (*ag)[2], (*ag)[3]}};
                                 }
 #pragma unroll
-                                for (int e = 0; e < 4; ++e)
-                                    hv.v[e] = nn_act_k<ACT, Real>(pv.v[e] + dp.v[e], gv.v[e] + dg.v[e]) - nn_act_k<ACT, Real>(pv.v[e], gv.v[e]);
-                                nn_st4<Real>(dpre + (int64_t)t * D + jc, dp);
-                                if (ag) nn_st4<Real>(dgpre + (int64_t)t * D + jc, dg);
+                                for (int e = 0; e < 4; ++e) {
+                                    nv.v[e] = pv.v[e] + dp.v[e];
+                                    ng.v[e] = gv.v[e] + dg.v[e];
+                                    hv.v[e] = nn_act_k<ACT, Real>(nv.v[e], ng.v[e]) - nn_act_k<ACT, Real>(pv.v[e], gv.v[e]);
+                                }
+                                nn_st4<Real>(npre + (int64_t)t * D + jc, nv);
+                                if (ag) nn_st4<Real>(ngpre + (int64_t)t * D + jc, ng);
                                 if (last) {
 #pragma unroll
                                     for (int e = 0; e < 4; ++e) part += (double)(hv.v[e] * hw[jc + e]);
@@ -1202,15 +1207,19 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     V4 ha, hb;
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        ha.v[e] = nn_act<Real>(m, pa.v[e] + dpa.v[e], ga.v[e] + dga.v[e]) - nn_act<Real>(m, pa.v[e], ga.v[e]);
-                        hb.v[e] = nn_act<Real>(m, pb.v[e] + dpb.v[e], gb.v[e] + dgb.v[e]) - nn_act<Real>(m, pb.v[e], gb.v[e]);
+                        dpa.v[e] += pa.v[e];                 // the new pre-activations
+                        dpb.v[e] += pb.v[e];
+                        dga.v[e] += ga.v[e];
+                        dgb.v[e] += gb.v[e];
+                        ha.v[e] = nn_act<Real>(m, dpa.v[e], dga.v[e]) - nn_act<Real>(m, pa.v[e], ga.v[e]);
+                        hb.v[e] = nn_act<Real>(m, dpb.v[e], dgb.v[e]) - nn_act<Real>(m, pb.v[e], gb.v[e]);
                     }
                     // scratch rows by target: item k covers elements 8k .. 8k + 7
-                    nn_st4<Real>(dpre + 8 * (int64_t)k, dpa);
-                    nn_st4<Real>(dpre + 8 * (int64_t)k + 4, dpb);
+                    nn_st4<Real>(npre + 8 * (int64_t)k, dpa);
+                    nn_st4<Real>(npre + 8 * (int64_t)k + 4, dpb);
                     if (m.has_gate) {
-                        nn_st4<Real>(dgpre + 8 * (int64_t)k, dga);
-                        nn_st4<Real>(dgpre + 8 * (int64_t)k + 4, dgb);
+                        nn_st4<Real>(ngpre + 8 * (int64_t)k, dga);
+                        nn_st4<Real>(ngpre + 8 * (int64_t)k + 4, dgb);
                     }
                     if (last) {
 #pragma unroll
@@ -1276,11 +1285,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         }
                         V4 dhv;
 #pragma unroll
-                        for (int e = 0; e < 4; ++e)
-                            dhv.v[e] = nn_act<Real>(m, p0[u].v[e] + dp.v[e], g0[u].v[e] + dg.v[e]) -
-                                       nn_act<Real>(m, p0[u].v[e], g0[u].v[e]);
-                        nn_st4<Real>(dpre + 4 * (int64_t)k, dp);
-                        if (m.has_gate) nn_st4<Real>(dgpre + 4 * (int64_t)k, dg);
+                        for (int e = 0; e < 4; ++e) {
+                            dp.v[e] += p0[u].v[e];               // the new pre-activations
+                            dg.v[e] += g0[u].v[e];
+                            dhv.v[e] = nn_act<Real>(m, dp.v[e], dg.v[e]) - nn_act<Real>(m, p0[u].v[e], g0[u].v[e]);
+                        }
+                        nn_st4<Real>(npre + 4 * (int64_t)k, dp);
+                        if (m.has_gate) nn_st4<Real>(ngpre + 4 * (int64_t)k, dg);
                         if (last) {
 #pragma unroll
                             for (int e = 0; e < 4; ++e) part += (double)(dhv.v[e] * hw[d + e]);
@@ -1346,11 +1357,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         }
                         V2 dhv;
 #pragma unroll
-                        for (int e = 0; e < 2; ++e)
-                            dhv.v[e] = nn_act<Real>(m, p0[u].v[e] + dp.v[e], g0[u].v[e] + dg.v[e]) -
-                                       nn_act<Real>(m, p0[u].v[e], g0[u].v[e]);
-                        nn_st2<Real>(dpre + 2 * (int64_t)k, dp);
-                        if (m.has_gate) nn_st2<Real>(dgpre + 2 * (int64_t)k, dg);
+                        for (int e = 0; e < 2; ++e) {
+                            dp.v[e] += p0[u].v[e];               // the new pre-activations
+                            dg.v[e] += g0[u].v[e];
+                            dhv.v[e] = nn_act<Real>(m, dp.v[e], dg.v[e]) - nn_act<Real>(m, p0[u].v[e], g0[u].v[e]);
+                        }
+                        nn_st2<Real>(npre + 2 * (int64_t)k, dp);
+                        if (m.has_gate) nn_st2<Real>(ngpre + 2 * (int64_t)k, dg);
                         if (last) {
 #pragma unroll
                             for (int e = 0; e < 2; ++e) part += (double)(dhv.v[e] * hw[d + e]);
@@ -1394,9 +1407,10 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             dp += pr[a + D];
                             if (m.has_gate) dg += pr[a + og];
                         }
-                        const Real dhv = nn_act<Real>(m, p0[u] + dp, g0[u] + dg) - nn_act<Real>(m, p0[u], g0[u]);
-                        dpre[k] = dp;
-                        if (m.has_gate) dgpre[k] = dg;
+                        const Real np = p0[u] + dp, ng = g0[u] + dg;   // the new pre-activations
+                        const Real dhv = nn_act<Real>(m, np, ng) - nn_act<Real>(m, p0[u], g0[u]);
+                        npre[k] = np;
+                        if (m.has_gate) ngpre[k] = ng;
                         if (last) part += (double)(dhv * hw[d]);
                         else dh[k] = dhv;
                     }
@@ -1426,15 +1440,32 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             if (eval_dE && tid == 0) eval_dE[blockIdx.x] = dE;
             if (accept) {
                 // (the parts of a split replica update alternate rows)
-                if (m.cache_h0)
-                    for (int k = tid; k < n_own(counts[0]) * C0; k += NN_BS) {
-                        const int so = k / C0, c = k - so * C0, s = prt + so * S;
-                        cache[(int64_t)slot_site[s] * C0 + c] += dh0[(int64_t)s * C0 + c];
+                // (NN_CU items per thread per round: every cache load of a round is issued before its first add, so
+                // the rounds, not the items, pay the memory latency)
+                if (m.cache_h0) {
+                    const int nk = n_own(counts[0]) * C0;
+                    for (int k0 = tid; k0 < nk; k0 += NN_CU * NN_BS) {
+                        int64_t o[NN_CU];
+                        Real cv[NN_CU], dv[NN_CU];
+#pragma unroll
+                        for (int u = 0; u < NN_CU; ++u) {
+                            const int k = min(k0 + u * NN_BS, nk - 1);
+                            const int so = k / C0, c = k - so * C0, s = prt + so * S;
+                            o[u] = (int64_t)slot_site[s] * C0 + c;
+                            dv[u] = dh0[(int64_t)s * C0 + c];
+                            cv[u] = cache[o[u]];
+                        }
+#pragma unroll
+                        for (int u = 0; u < NN_CU; ++u)
+                            if (k0 + u * NN_BS < nk) cache[o[u]] = cv[u] + dv[u];
                     }
+                }
                 for (int l = 0; l < L; ++l) {
                     const int D = m.dd[l];
-                    const NCE_GLOBAL Real *dpre = scr + m.soff_dpre[l];
-                    const NCE_GLOBAL Real *dgpre = scr + m.soff_dgpre[l];
+                    // the new pre-activations of the step's targets replace the cached rows (the same FP32 or FP64
+                    // sums the layers formed)
+                    const NCE_GLOBAL Real *npre = scr + m.soff_npre[l];
+                    const NCE_GLOBAL Real *ngpre = scr + m.soff_ngpre[l];
                     if (sizeof(Real) == 4 && (D & 3) == 0) {
                         // four channels per thread: 16-byte rows of the scratch and the cache
                         using V4 = NNv4<Real>;
@@ -1442,24 +1473,44 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         const int d4sh = (D4 & (D4 - 1)) == 0 ? __builtin_ctz((unsigned)D4) : -1;
                         for (int g = 0; g < (m.has_gate ? 2 : 1); ++g) {
                             NCE_GLOBAL Real *cbase = cache + (g == 0 ? m.off_pre[l] : m.off_gpre[l]);
-                            const NCE_GLOBAL Real *dbase = g == 0 ? dpre : dgpre;
-                            for (int k = tid; k < n_own(counts[l + 1]) * D4; k += NN_BS) {
-                                const int so = d4sh >= 0 ? k >> d4sh : k / D4, d = (k - so * D4) * 4, s = prt + so * S;
-                                NCE_GLOBAL Real *cp = cbase + (int64_t)slot_site[s] * D + d;
-                                const V4 dv = nn_ld4<Real>(dbase + (int64_t)s * D + d);
-                                V4 c = nn_ld4<Real>(cp);
+                            const NCE_GLOBAL Real *nbase = g == 0 ? npre : ngpre;
+                            const int nk = n_own(counts[l + 1]) * D4;
+                            for (int k0 = tid; k0 < nk; k0 += NN_CU * NN_BS) {
+                                NCE_GLOBAL Real *cp[NN_CU];
+                                V4 nv[NN_CU];
 #pragma unroll
-                                for (int e = 0; e < 4; ++e) c.v[e] += dv.v[e];
-                                nn_st4<Real>(cp, c);
+                                for (int u = 0; u < NN_CU; ++u) {
+                                    const int k = min(k0 + u * NN_BS, nk - 1);
+                                    const int so = d4sh >= 0 ? k >> d4sh : k / D4, d = (k - so * D4) * 4;
+                                    const int s = prt + so * S;
+                                    cp[u] = cbase + (int64_t)slot_site[s] * D + d;
+                                    nv[u] = nn_ld4<Real>(nbase + (int64_t)s * D + d);
+                                }
+#pragma unroll
+                                for (int u = 0; u < NN_CU; ++u)
+                                    if (k0 + u * NN_BS < nk) nn_st4<Real>(cp[u], nv[u]);
                             }
                         }
                     } else {
                         const int dsh = (D & (D - 1)) == 0 ? __builtin_ctz((unsigned)D) : -1;
-                        for (int k = tid; k < n_own(counts[l + 1]) * D; k += NN_BS) {
-                            const int so = dsh >= 0 ? k >> dsh : k / D, d = k - so * D, s = prt + so * S;
-                            const int64_t o = (int64_t)slot_site[s] * D + d;
-                            cache[m.off_pre[l] + o] += dpre[(int64_t)s * D + d];
-                            if (m.has_gate) cache[m.off_gpre[l] + o] += dgpre[(int64_t)s * D + d];
+                        const int nk = n_own(counts[l + 1]) * D;
+                        for (int g = 0; g < (m.has_gate ? 2 : 1); ++g) {
+                            NCE_GLOBAL Real *cbase = cache + (g == 0 ? m.off_pre[l] : m.off_gpre[l]);
+                            const NCE_GLOBAL Real *nbase = g == 0 ? npre : ngpre;
+                            for (int k0 = tid; k0 < nk; k0 += NN_CU * NN_BS) {
+                                int64_t o[NN_CU];
+                                Real nv[NN_CU];
+#pragma unroll
+                                for (int u = 0; u < NN_CU; ++u) {
+                                    const int k = min(k0 + u * NN_BS, nk - 1);
+                                    const int so = dsh >= 0 ? k >> dsh : k / D, d = k - so * D, s = prt + so * S;
+                                    o[u] = (int64_t)slot_site[s] * D + d;
+                                    nv[u] = nbase[(int64_t)s * D + d];
+                                }
+#pragma unroll
+                                for (int u = 0; u < NN_CU; ++u)
+                                    if (k0 + u * NN_BS < nk) cbase[o[u]] = nv[u];
+                            }
                         }
                     }
                 }
@@ -1843,8 +1894,8 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     m.soff_pub = s2; if (m.fused_ok) s2 = pad4(s2 + m.L + 1 + 2 * (int64_t)m.W0 + m.ball_cap[m.L]);
     m.soff_list = s2; s2 = pad4(s2 + hit_max);
     for (int l = 0; l < m.L; ++l) {
-        m.soff_dpre[l] = s2; s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
-        m.soff_dgpre[l] = s2; if (m.has_gate) s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
+        m.soff_npre[l] = s2; s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
+        m.soff_ngpre[l] = s2; if (m.has_gate) s2 = pad4(s2 + (int64_t)m.ball_cap[l + 1] * m.dd[l]);
     }
     m.soff_dh[0] = s2; s2 = pad4(s2 + dh_max);
     m.soff_dh[1] = s2; s2 = pad4(s2 + dh_max);
