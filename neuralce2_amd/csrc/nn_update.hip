@@ -1100,15 +1100,25 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             using V4 = NNv4<Real>;
                             const V4 z4 = V4{{(Real)0, (Real)0, (Real)0, (Real)0}};
                             const int64_t so = (int64_t)slot_site[t] * D;
+                            // the cached rows of every output quad of the group first: a store to the scratch ahead
+                            // of a cache load would keep the load from moving up (the pointers may alias), so each
+                            // quad would wait on the memory in turn (a quad past the last reads channels in range)
+                            constexpr int NQ = NN_FCT;
+                            V4 pq[NQ], gq[NQ];
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                const bool gate = m.has_gate;
+                                if (gate && q >= NQ / 2) break;
+                                const int jc = min(((gate ? c0t / 2 : c0t) + q) * 16 + 4 * g, D - 4);
+                                pq[q] = nn_ld4<Real>(cpre + so + jc);
+                                gq[q] = gate ? nn_ld4<Real>(cgpre + so + jc) : z4;
+                            }
                             // four output channels jc .. jc + 3 of target t (ACT: the activation, fixed per tile)
-                            auto out = [&]<int ACT>(int jc, const nn_floatx4 &ap, const nn_floatx4 *ag) {
-                                const V4 pv = nn_ld4<Real>(cpre + so + jc);
+                            auto out = [&]<int ACT>(int jc, const nn_floatx4 &ap, const nn_floatx4 *ag, const V4 &pv,
+                                                    const V4 &gv) {
                                 const V4 dp = V4{{ap[0], ap[1], ap[2], ap[3]}};
-                                V4 gv = z4, dg = z4, hv, nv, ng;
-                                if (ag) {
-                                    gv = nn_ld4<Real>(cgpre + so + jc);
-                                    dg = V4{{(*ag)[0], (*ag)[1], (*ag)[2], (*ag)[3]}};
-                                }
+                                V4 dg = z4, hv, nv, ng;
+                                if (ag) dg = V4{{(*ag)[0], (*ag)[1], (*ag)[2], (*ag)[3]}};
 #pragma unroll
                                 for (int e = 0; e < 4; ++e) {
                                     nv.v[e] = pv.v[e] + dp.v[e];
@@ -1132,15 +1142,18 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
 #pragma unroll
                                 for (int ct = 0; ct < NN_FCT / 2; ++ct)
                                     if (c0t + 2 * ct < nct)
-                                        out.template operator()<2>((c0t / 2 + ct) * 16 + 4 * g, acc[2 * ct], &acc[2 * ct + 1]);
+                                        out.template operator()<2>((c0t / 2 + ct) * 16 + 4 * g, acc[2 * ct], &acc[2 * ct + 1],
+                                                                   pq[ct], gq[ct]);
                             } else if (m.act == 1) {
 #pragma unroll
                                 for (int ct = 0; ct < NN_FCT; ++ct)
-                                    if (c0t + ct < nct) out.template operator()<1>((c0t + ct) * 16 + 4 * g, acc[ct], nullptr);
+                                    if (c0t + ct < nct)
+                                        out.template operator()<1>((c0t + ct) * 16 + 4 * g, acc[ct], nullptr, pq[ct], z4);
                             } else {
 #pragma unroll
                                 for (int ct = 0; ct < NN_FCT; ++ct)
-                                    if (c0t + ct < nct) out.template operator()<0>((c0t + ct) * 16 + 4 * g, acc[ct], nullptr);
+                                    if (c0t + ct < nct)
+                                        out.template operator()<0>((c0t + ct) * 16 + 4 * g, acc[ct], nullptr, pq[ct], z4);
                             }
                             // a width that is not a multiple of 32: the next layer's padding channels D .. D + 15 are 0
                             if (!last && (D & 31) && c0t == 0)
