@@ -260,6 +260,7 @@ __global__ void nn_energy_finish_kernel(const NNModelDev *__restrict__ models, i
 constexpr int NN_BS = 1024;                // threads per replica: 16 waves, 4 per SIMD
 constexpr int NN_EMPTY = -1;
 constexpr int NN_RB = 4;                 // rows per thread in the projection
+constexpr int NN_EU = 4;                 // layer-0 entries per thread in flight together
 constexpr int NN_CU = 4;                 // cache rows per thread in flight together at a commit
 constexpr int NN_LA = 8;                 // proposals drawn, and their species read, ahead of the steps (power of 2)
 // one drawn proposal and the species of its two sites at the step it runs
@@ -688,37 +689,50 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = vals[w_pos[k]];
             }
             __syncthreads();
-            // layer-0 change of flip 1, then of flip 2 with flip 1 applied
-            for (int f = 0; f < 2; ++f) {
-                const int64_t fs = f == 0 ? ia : ib;
-                const int wb = f == 0 ? 0 : m.W0;          // this flip's window in w_site / w_pos / w_sp
-                for (int e = tid; e < m.J0; e += NN_BS) {
-                    const uint64_t sl = ent_slot[e];
-                    const uint32_t lo = (uint32_t)sl, hi = (uint32_t)(sl >> 32);
+            // layer-0 change of flip 1, then of flip 2 with flip 1 applied.  A round loads NN_EU entries per thread
+            // (slots, coefficient, channel) before it uses any, and both flips use them.
+            for (int e0 = tid; e0 < m.J0; e0 += NN_EU * NN_BS) {
+                uint64_t esl[NN_EU];
+                double eq[NN_EU];
+                int ecol[NN_EU];
+#pragma unroll
+                for (int u = 0; u < NN_EU; ++u) {
+                    const int e = min(e0 + u * NN_BS, m.J0 - 1);
+                    esl[u] = ent_slot[e];
+                    eq[u] = ent_q[e];
+                    ecol[u] = ent_col[e];
+                }
+#pragma unroll
+                for (int u = 0; u < NN_EU; ++u) {
+                    if (e0 + u * NN_BS >= m.J0) break;
+                    const uint32_t lo = (uint32_t)esl[u], hi = (uint32_t)(esl[u] >> 32);
                     const uint32_t s4[4] = {lo & 0xffffu, lo >> 16, hi & 0xffffu, hi >> 16};
-                    int xs[4];
-                    double po = 1.0, pn = 1.0;
-                    for (int q = 0; q < 4; ++q) {
-                        if (s4[q] == 0xffffu) { xs[q] = -1; continue; }
-                        const int x = w_site[wb + s4[q]];
-                        xs[q] = wb + (int)s4[q];
-                        int sp = w_sp[wb + s4[q]];
-                        if (f == 1 && x == ia) sp = sj;           // flip 2 sees flip 1 applied
-                        int spn = sp;
-                        if (x == fs) spn = f == 0 ? sj : si;
-                        po *= sp ? 1.0 : -1.0;
-                        pn *= spn ? 1.0 : -1.0;
-                    }
-                    // (pn - po) is 0 or +-2, so (pn - po) * (coef / sd) rounds as ((pn - po) * coef) / sd does
-                    const double dv = (pn - po) * ent_q[e];
-                    if (dv != 0.0) {
-                        const int col = ent_col[e];
-                        const Real v = (Real)dv;
+                    for (int f = 0; f < 2; ++f) {
+                        const int64_t fs = f == 0 ? ia : ib;
+                        const int wb = f == 0 ? 0 : m.W0;      // this flip's window in w_site / w_pos / w_sp
+                        int xs[4];
+                        double po = 1.0, pn = 1.0;
                         for (int q = 0; q < 4; ++q) {
-                            if (xs[q] < 0) continue;
-                            const int s = w_pos[xs[q]];
-                            if ((s & (S - 1)) != prt) continue;   // a split replica's part keeps its own rows
-                            atomicAdd((Real *)&dh0[(int64_t)s * C0 + col], v);
+                            if (s4[q] == 0xffffu) { xs[q] = -1; continue; }
+                            const int x = w_site[wb + s4[q]];
+                            xs[q] = wb + (int)s4[q];
+                            int sp = w_sp[wb + s4[q]];
+                            if (f == 1 && x == ia) sp = sj;           // flip 2 sees flip 1 applied
+                            int spn = sp;
+                            if (x == fs) spn = f == 0 ? sj : si;
+                            po *= sp ? 1.0 : -1.0;
+                            pn *= spn ? 1.0 : -1.0;
+                        }
+                        // (pn - po) is 0 or +-2, so (pn - po) * (coef / sd) rounds as ((pn - po) * coef) / sd does
+                        const double dv = (pn - po) * eq[u];
+                        if (dv != 0.0) {
+                            const Real v = (Real)dv;
+                            for (int q = 0; q < 4; ++q) {
+                                if (xs[q] < 0) continue;
+                                const int s = w_pos[xs[q]];
+                                if ((s & (S - 1)) != prt) continue;   // a split replica's part keeps its own rows
+                                atomicAdd((Real *)&dh0[(int64_t)s * C0 + ecol[u]], v);
+                            }
                         }
                     }
                 }
