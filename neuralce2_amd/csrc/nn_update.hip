@@ -1013,9 +1013,9 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 auto layer = [&]<int NSB>() {
                     constexpr int NZ = 8 * NSB;
                     const uint32_t cob = 16u * (uint32_t)g;   // byte offset of the lane's channels in step 0
-                    // (the parts of a split replica interleave, so a layer's last, partial round of tiles is spread over
-                // all of them)
-                for (int task = wave * S + prt; task < ntask; task += S * (NN_BS / 64)) {
+                    // (the parts of a split replica interleave, so a layer's last, partial round of tiles is spread
+                    // over all of them)
+                    for (int task = wave * S + prt; task < ntask; task += S * (NN_BS / 64)) {
                         const int tile = task / ngrp, c0t = (task - tile * ngrp) * NN_FCT;
                         const int t = tile * 16 + tl;
                         const bool tv = t < nb;
