@@ -2153,6 +2153,9 @@ static int nn_launch(nce_ctx *c, const nce_nn_job *jobs, int32_t n_jobs, int64_t
         if (n_items % 8 == 0 && c->n_cu % 8 == 0)
             for (int cand = 8; cand >= 2; cand >>= 1)
                 if (cand <= smax && n_items * cand <= c->n_cu) { S = cand; break; }
+        // (only with the layer-0 change in LDS: in the scratch, one array per replica, the parts' clearing of it
+        // would race with one another's adds)
+        if (mode < 1) S = 1;
         int coop = 0;
         if (S > 1) HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device));
         if (S > 1 && coop) {
