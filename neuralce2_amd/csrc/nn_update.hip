@@ -1085,11 +1085,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                                     auto rsel = [&](uint32_t v) -> int { return (int)v < prev ? (int)v : zrow; };
                                     const int rw[4] = {rsel(e4[cc].x & 0xffffu), rsel(e4[cc].x >> 16),
                                                        rsel(e4[cc].y & 0xffffu), rsel(e4[cc].y >> 16)};
+                                    // (a column without a partner adds the zero row: its lanes load nothing)
                                     nn_bf16x8 v[4][NSB];
 #pragma unroll
                                     for (int u = 0; u < 4; ++u)
 #pragma unroll
-                                        for (int s = 0; s < NSB; ++s) v[u][s] = ldrow(rw[u], s);
+                                        for (int s = 0; s < NSB; ++s)
+                                            v[u][s] = rw[u] != zrow ? ldrow(rw[u], s) : nn_bf16x8{};
 #pragma unroll
                                     for (int u = 0; u < 4; ++u)
 #pragma unroll
