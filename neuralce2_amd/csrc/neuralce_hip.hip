@@ -133,6 +133,7 @@ struct NNModelDev {
     double head_b, de_scale;
     int32_t ball_cap[NN_MAXL + 1];
     int32_t hash_cap;            // power of two >= 1.5 * ball_cap[L]
+    int32_t hash_cap_f;          // the BF16 layers' table (uint16 slots, no hit counts): as large as LDS allows
     // Two flips whose last balls share no site (ib - ia outside ball_L - ball_L): the union's slots, hits and
     // target ranges are then the same for every such pair, so they are tables built once for centre 0.
     // pre_ok = 0: every step builds its balls in the hash table.

@@ -273,13 +273,14 @@ struct NNLds {
     size_t keys, vals, slot_site, cnt, red, misc, la, wv, part, w, dh0, list, pr, total;
 };
 __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int win_n, int n_part, int n_c0, size_t w_bytes,
-                                               size_t dh0_bytes, size_t list_bytes, size_t pr_bytes) {
+                                               size_t dh0_bytes, size_t list_bytes, size_t pr_bytes,
+                                               size_t val_bytes = sizeof(int), bool with_cnt = true) {
     NNLds L;
     size_t o = 0;
     L.keys = o;      o = align16(o + (size_t)hash_cap * sizeof(int));
-    L.vals = o;      o = align16(o + (size_t)hash_cap * sizeof(int));
+    L.vals = o;      o = align16(o + (size_t)hash_cap * val_bytes);
     L.slot_site = o; o = align16(o + (size_t)ball_cap * sizeof(int));
-    L.cnt = o;       o = align16(o + (size_t)(ball_cap + 1) * sizeof(int));
+    L.cnt = o;       o = align16(o + (with_cnt ? (size_t)(ball_cap + 1) * sizeof(int) : 0));
     L.red = o;       o = align16(o + (NN_BS / 64 + 1) * sizeof(double));
     L.misc = o;      o = align16(o + (4 + NN_MAXL + 1) * sizeof(int));
     L.la = o;        o = align16(o + NN_LA * sizeof(NNProp));
@@ -293,14 +294,15 @@ __host__ __device__ inline NNLds nn_lds_layout(int hash_cap, int ball_cap, int w
     return L;
 }
 // fused: the BF16 layers, whose stacked matrix is staged instead of the projection matrix and which keep neither a
-// hit list nor a projection block in LDS
+// hit list nor a projection block in LDS; their hash table (hash_cap_f) holds uint16 slots and no hit counts
 template <typename Real>
 __host__ __device__ inline NNLds nn_lds_for(const NNModelDev &m, int mode, bool fused = false) {
-    return nn_lds_layout(m.hash_cap, m.ball_cap[m.L], 2 * m.W0, m.n_part, m.c0_eff,
+    return nn_lds_layout(fused ? m.hash_cap_f : m.hash_cap, m.ball_cap[m.L], 2 * m.W0, m.n_part, m.c0_eff,
                          fused ? (size_t)m.wf_bytes : (size_t)m.w_elems * sizeof(Real),
                          mode >= 1 ? (size_t)m.ball_cap[0] * m.C0 * sizeof(Real) : 0,
                          mode >= 2 && !fused ? (size_t)m.hit_elems * sizeof(int) : 0,
-                         mode >= 3 && !fused ? (size_t)m.pr_elems * sizeof(Real) : 0);
+                         mode >= 3 && !fused ? (size_t)m.pr_elems * sizeof(Real) : 0,
+                         fused ? sizeof(uint16_t) : sizeof(int), !fused);
 }
 
 // multiplicative hashing: the position is the top log2(cap) bits of the 32-bit product (cap = mask + 1 >= 64, a power
@@ -321,11 +323,12 @@ __device__ inline int nn_hash_claim(int *keys, int mask, int site, bool *claimed
     }
 }
 
-__device__ inline int nn_hash_find(const int *keys, const int *vals, int mask, int site) {
+template <typename V>
+__device__ inline int nn_hash_find(const int *keys, const V *vals, int mask, int site) {
     uint32_t h = nn_hash(site, mask);
     while (true) {
         const int k = keys[h];
-        if (k == site) return vals[h];
+        if (k == site) return (int)vals[h];
         if (k == NN_EMPTY) return -1;
         h = (h + 1) & (uint32_t)mask;
     }
@@ -456,10 +459,13 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     const int r = it.y;
     const int tid = threadIdx.x;
     const int L = m.L, C0 = m.C0, n_part = m.n_part, P = m.P;
-    const int mask = m.hash_cap - 1;
+    // hash table: the BF16 layers' one is larger (shorter probe chains) and holds uint16 slots
+    const int hcap = MFMA ? m.hash_cap_f : m.hash_cap;
+    const int mask = hcap - 1;
     const NNLds lay = nn_lds_for<Real>(m, MODE, MFMA);
+    using HV = typename std::conditional<MFMA, uint16_t, int>::type;
     int *keys = (int *)(smem + lay.keys);
-    int *vals = (int *)(smem + lay.vals);
+    HV *vals = (HV *)(smem + lay.vals);
     int *slot_site = (int *)(smem + lay.slot_site);
     int *cnt = (int *)(smem + lay.cnt);
     double *red = (double *)(smem + lay.red);
@@ -477,7 +483,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
         s_part_col[k] = m.part_col[k];
     }
     for (int k = tid; k < m.c0_eff; k += NN_BS) s_c0[k] = m.c0_ch[k];
-    for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
+    for (int i = tid; i < hcap; i += NN_BS) keys[i] = NN_EMPTY;
     if (tid == 0) {
         misc[0] = 0;
         misc[1] = 0;                                   // abort flag of a split replica
@@ -600,7 +606,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const int pos = nn_hash_claim(keys, mask, site, &claimed);
                     if (claimed) {
                         const int s = atomicAdd(&misc[0], 1);
-                        vals[pos] = s;
+                        vals[pos] = (HV)s;
                         slot_site[s] = site;
                     }
                     w_pos[k] = pos;
@@ -638,7 +644,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             const int pos = nn_hash_claim(keys, mask, site, &claimed);
                             if (claimed) {
                                 const int sl = atomicAdd(&misc[0], 1);
-                                vals[pos] = sl;
+                                vals[pos] = (HV)sl;
                                 slot_site[sl] = site;
                             }
                         }
@@ -946,7 +952,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const int pos = nn_hash_claim(keys, mask, site, &claimed);
                     if (claimed) {
                         const int s = atomicAdd(&misc[0], 1);
-                        vals[pos] = s;
+                        vals[pos] = (HV)s;
                         slot_site[s] = site;
                     }
                     tgt[i] = pos;
@@ -1558,7 +1564,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             }
             // clear the table for the next step (a step on the fixed tables leaves it empty)
             if (!fixed) {
-                for (int i = tid; i < m.hash_cap; i += NN_BS) keys[i] = NN_EMPTY;
+                for (int i = tid; i < hcap; i += NN_BS) keys[i] = NN_EMPTY;
                 if (tid == 0) misc[0] = 0;
             }
         }
@@ -1875,6 +1881,7 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     int hc = 64;
     while (hc < m.ball_cap[m.L] + m.ball_cap[m.L] / 2) hc <<= 1;
     m.hash_cap = hc;
+    m.hash_cap_f = hc;                           // grown below once the fused layout's placement is known
     // per-replica cache layout; every region starts at a multiple of 4 elements (16-byte rows of four channels)
     auto pad4 = [](int64_t v) { return (v + 3) & ~(int64_t)3; };
     int64_t o = m.cache_h0 ? pad4(N * m.C0) : 0;
@@ -1973,6 +1980,14 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     m.lds_mode32 = nn_lds_mode<float>(m);
     m.lds_mode64 = nn_lds_mode<double>(m);
     m.lds_mode16 = nn_lds_mode<float>(m, true);
+    // the BF16 layers' hash table: doubled while the fused layout keeps its placement (a wave's claims and lookups
+    // wait for its longest probe chain: ~2.3 probes at 56 % load against ~1.1 at 28 %), up to 4 ball_cap[L]
+    while (m.hash_cap_f < 4 * m.ball_cap[m.L]) {
+        NNModelDev t = m;
+        t.hash_cap_f = 2 * m.hash_cap_f;
+        if (nn_lds_for<float>(t, m.lds_mode16, true).total > 160 * 1024) break;
+        m.hash_cap_f = t.hash_cap_f;
+    }
     m.fp32_in_lds = m.lds_mode32 == 3;
     if (m.fused_ok && nn_lds_for<float>(m, 0, true).total > 160 * 1024) m.fused_ok = 0;
     // NCE_NN_LDS_MODE_MAX=k (0..3) at model creation keeps at most the first k per-step arrays in LDS (the others in
