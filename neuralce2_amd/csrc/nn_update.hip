@@ -470,7 +470,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     int *cnt = (int *)(smem + lay.cnt);
     double *red = (double *)(smem + lay.red);
     int *misc = (int *)(smem + lay.misc);
-    // the two layer-0 windows of a step: site, hash position and species of every window slot
+    // the two layer-0 windows of a step: site, hash position (then slot) and species of every window slot (w_sp: bit 0
+    // the species before the slot's flip, bit 1 after it; the second window sees the first flip applied)
     int *w_site = (int *)(smem + lay.wv), *w_pos = w_site + 2 * m.W0, *w_sp = w_pos + 2 * m.W0;
     // shell partners (offset and shell of each) and the used layer-0 channels, staged once per launch
     int4 *s_part = (int4 *)(smem + lay.part);
@@ -612,7 +613,12 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     w_pos[k] = pos;
                 }
                 w_site[k] = site;
-                w_sp[k] = occ[site];
+                const bool f2 = k >= m.W0;
+                int sp = occ[site];
+                if (f2 && site == ia) sp = sj;                // flip 2 sees flip 1 applied
+                int spn = sp;
+                if (site == (f2 ? ib : ia)) spn = f2 ? si : sj;
+                w_sp[k] = (sp ? 1 : 0) | (spn ? 2 : 0);
             }
             if (fixed) {
                 const NCE_GLOBAL int *ps = (const NCE_GLOBAL int *)gptr(m.pre_slot);
@@ -714,20 +720,15 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     const uint32_t lo = (uint32_t)esl[u], hi = (uint32_t)(esl[u] >> 32);
                     const uint32_t s4[4] = {lo & 0xffffu, lo >> 16, hi & 0xffffu, hi >> 16};
                     for (int f = 0; f < 2; ++f) {
-                        const int64_t fs = f == 0 ? ia : ib;
-                        const int wb = f == 0 ? 0 : m.W0;      // this flip's window in w_site / w_pos / w_sp
+                        const int wb = f == 0 ? 0 : m.W0;      // this flip's window in w_pos / w_sp
                         int xs[4];
                         double po = 1.0, pn = 1.0;
                         for (int q = 0; q < 4; ++q) {
                             if (s4[q] == 0xffffu) { xs[q] = -1; continue; }
-                            const int x = w_site[wb + s4[q]];
                             xs[q] = wb + (int)s4[q];
-                            int sp = w_sp[wb + s4[q]];
-                            if (f == 1 && x == ia) sp = sj;           // flip 2 sees flip 1 applied
-                            int spn = sp;
-                            if (x == fs) spn = f == 0 ? sj : si;
-                            po *= sp ? 1.0 : -1.0;
-                            pn *= spn ? 1.0 : -1.0;
+                            const int b = w_sp[xs[q]];         // species before and after this flip
+                            po *= (b & 1) ? 1.0 : -1.0;
+                            pn *= (b & 2) ? 1.0 : -1.0;
                         }
                         // (pn - po) is 0 or +-2, so (pn - po) * (coef / sd) rounds as ((pn - po) * coef) / sd does
                         const double dv = (pn - po) * eq[u];
