@@ -471,7 +471,8 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
     double *red = (double *)(smem + lay.red);
     int *misc = (int *)(smem + lay.misc);
     // the two layer-0 windows of a step: site, hash position (then slot) and species of every window slot (w_sp: bit 0
-    // the species before the slot's flip, bit 1 after it; the second window sees the first flip applied)
+    // the species before the slot's flip, bit 1 after it, the slot above them; the second window sees the first flip
+    // applied)
     int *w_site = (int *)(smem + lay.wv), *w_pos = w_site + 2 * m.W0, *w_sp = w_pos + 2 * m.W0;
     // shell partners (offset and shell of each) and the used layer-0 channels, staged once per launch
     int4 *s_part = (int4 *)(smem + lay.part);
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                 if (f2 && site == ia) sp = sj;                // flip 2 sees flip 1 applied
                 int spn = sp;
                 if (site == (f2 ? ib : ia)) spn = f2 ? si : sj;
-                w_sp[k] = (sp ? 1 : 0) | (spn ? 2 : 0);
+                w_sp[k] = (sp ? 1 : 0) | (spn ? 2 : 0) | (fixed ? w_pos[k] << 2 : 0);
             }
             if (fixed) {
                 const NCE_GLOBAL int *ps = (const NCE_GLOBAL int *)gptr(m.pre_slot);
@@ -634,7 +635,10 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                     for (int l = 0; l <= L; ++l) counts[l] = 2 * m.pre_n[l];
                 } else if (builder) {
                     counts[0] = misc[0];
-                    for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = vals[w_pos[k]];
+                    for (int k = tid; k < 2 * m.W0; k += NN_BS) {
+                        w_pos[k] = vals[w_pos[k]];
+                        w_sp[k] |= w_pos[k] << 2;
+                    }
                     // hit i = (q, kk) = (i / n_part, i % n_part), stepped without divisions
                     const int q_0 = n_part > 0 ? tid / n_part : 0, kk_0 = tid - q_0 * n_part;
                     const int dq = n_part > 0 ? NN_BS / n_part : 0, dk = NN_BS - dq * n_part;
@@ -687,7 +691,10 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                         // (clamped into range, so that rows of an aborted step still address memory in bounds)
                         for (int k = tid; k <= L; k += NN_BS) counts[k] = max(0, min(pub[k], m.ball_cap[k]));
                         __syncthreads();
-                        for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = max(0, min(pub[L + 1 + k], counts[0] - 1));
+                        for (int k = tid; k < 2 * m.W0; k += NN_BS) {
+                            w_pos[k] = max(0, min(pub[L + 1 + k], counts[0] - 1));
+                            w_sp[k] |= w_pos[k] << 2;
+                        }
                         for (int k = tid; k < counts[L]; k += NN_BS)
                             slot_site[k] = max(0, min(pub[L + 1 + 2 * m.W0 + k], (int)m.n_sites - 1));
                     }
@@ -698,7 +705,10 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
             counts[0] = fixed ? 2 * m.pre_n[0] : misc[0];
             for (int k = tid; k < counts[0] * C0; k += NN_BS) dh0[k] = (Real)0;
             if (!fixed)
-                for (int k = tid; k < 2 * m.W0; k += NN_BS) w_pos[k] = vals[w_pos[k]];
+                for (int k = tid; k < 2 * m.W0; k += NN_BS) {
+                    w_pos[k] = vals[w_pos[k]];
+                    w_sp[k] |= w_pos[k] << 2;
+                }
             }
             __syncthreads();
             // layer-0 change of flip 1, then of flip 2 with flip 1 applied.  A round loads NN_EU entries per thread
@@ -736,7 +746,7 @@ __global__ __launch_bounds__(NN_BS) void nn_chunk_kernel(const NNModelDev *__res
                             const Real v = (Real)dv;
                             for (int q = 0; q < 4; ++q) {
                                 if (xs[q] < 0) continue;
-                                const int s = w_pos[xs[q]];
+                                const int s = w_sp[xs[q]] >> 2;   // the member's slot
                                 if ((s & (S - 1)) != prt) continue;   // a split replica's part keeps its own rows
                                 atomicAdd((Real *)&dh0[(int64_t)s * C0 + ecol[u]], v);
                             }
