@@ -1742,15 +1742,33 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     m.c0_eff = m.L > 0 ? (int32_t)c0_ch.size() : 0;
     const int DL = m.L > 0 ? m.dd[m.L - 1] : 0;
     // slots
-    std::vector<uint2> slots(m.J0);
+    // the entries interleaved over the channels (the i-th entry of every channel, then the (i+1)-th, ...): every
+    // entry adds to the row of the flip's own site, and consecutive entries of one channel would make the lanes of
+    // one atomic add meet at one address.  A channel's entries keep their order, so the full recompute, which adds
+    // them in table order, sums them as before.
+    for (int e = 0; e < m.J0; ++e)
+        if (d->entry_col[e] < 1 || d->entry_col[e] >= m.C0) return set_err(NCE_ERR_ARG, "entry channel out of range");
+    std::vector<int> eord(m.J0), erank(m.J0), eseen(m.C0, 0);
     for (int e = 0; e < m.J0; ++e) {
+        eord[e] = e;
+        erank[e] = eseen[d->entry_col[e]]++;
+    }
+    std::stable_sort(eord.begin(), eord.end(), [&](int a, int b) {
+        return erank[a] != erank[b] ? erank[a] < erank[b] : d->entry_col[a] < d->entry_col[b];
+    });
+    std::vector<uint2> slots(m.J0);
+    std::vector<int32_t> ent_col_s(std::max(m.J0, 1), 0);
+    std::vector<double> ent_coef_s(std::max(m.J0, 1), 0.0);
+    for (int i = 0; i < m.J0; ++i) {
+        const int e = eord[i];
         uint32_t q[4];
         for (int k = 0; k < 4; ++k) {
             q[k] = d->entry_slot[(size_t)e * 4 + k];
             if (q[k] != 0xFFFFu && (int)q[k] >= m.W0) return set_err(NCE_ERR_ARG, "entry slot outside the window");
         }
-        if (d->entry_col[e] < 1 || d->entry_col[e] >= m.C0) return set_err(NCE_ERR_ARG, "entry channel out of range");
-        slots[e] = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
+        slots[i] = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
+        ent_col_s[i] = d->entry_col[e];
+        ent_coef_s[i] = d->entry_coef[e];
     }
     auto chk4 = [&](const int32_t *w, int n) {
         for (int i = 0; i < n; ++i)
@@ -1954,10 +1972,10 @@ extern "C" int nce_nn_model_create(nce_ctx *c, const nce_nn_model_desc *d, int32
     for (int i = 0; i < DL; ++i) { hw[i] = d->head_w[i]; hw_f[i] = (float)d->head_w[i]; }
     UP((const int4 *)d->window, (size_t)m.W0, &m.win);
     UP(slots.data(), slots.size(), &m.ent_slot);
-    UP(d->entry_col, (size_t)m.J0, &m.ent_col);
-    UP(d->entry_coef, (size_t)m.J0, &m.ent_coef);
+    UP(ent_col_s.data(), (size_t)m.J0, &m.ent_col);
+    UP(ent_coef_s.data(), (size_t)m.J0, &m.ent_coef);
     std::vector<double> ent_q(std::max(m.J0, 1), 0.0);
-    for (int e = 0; e < m.J0; ++e) ent_q[e] = d->entry_coef[e] / d->sd[d->entry_col[e]];
+    for (int e = 0; e < m.J0; ++e) ent_q[e] = ent_coef_s[e] / d->sd[ent_col_s[e]];
     UP(ent_q.data(), ent_q.size(), &m.ent_q);
     std::vector<int32_t> sc(std::max(m.P, 1), 0);
     for (int p = 0; p < m.P; ++p) sc[p] = d->shell_count[p];
